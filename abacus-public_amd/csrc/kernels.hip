@@ -1,0 +1,736 @@
+// Device kernels of the near-field direct path, hand-written for gfx950.
+//
+//   k_gather_sinks / k_gather_sources   pencil gathers from device-resident
+//       PosXYZ slabs into padded float4 pencil streams (the device-side form of
+//       SinkPencilPlan/SourcePencilPlan::copy_into_pinned_memory,
+//       PencilPlan.cc:22-90, 214-255), plus a bounding box per 64-source block.
+//   k_plain      one 64-sink block per workgroup, one launch per SIC: the
+//       structure of ComputeDirects (PencilKernel.cu:97-189) with the
+//       reference's pair forms (DirectGPUKernels.cuh) and BLOCK_ACCUM tiles.
+//   k_coadd_plain  FP32 co-add of the W partials per sink in the reference's
+//       k order (PencilPlan.cc:140-207 as driven by PencilTask.cu:237-247).
+//   k_batched    persistent kernel draining a work queue sorted by cost;
+//       TwoSum-compensated FP32 per-pencil partials stored as FP64.
+//   k_coadd_f64  FP64 reduction of the W partials per sink.
+//
+// Pair kernel, all forms: d = source - sink, a -= g(r) d, w += max(0, b2 - r^2).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "nf_internal.h"
+
+namespace nf {
+namespace {
+
+constexpr int kWave = 64;
+constexpr float kFar = 1.0e30f;  // x of a padding source: its pair terms are exact zeros
+
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+
+// ---------------------------------------------------------------------------
+// Gathers
+// ---------------------------------------------------------------------------
+
+// Position of padded index i within a pencil: cell c with pre[c] <= i < pre[c+1].
+__device__ __forceinline__ int cell_of(int i, const int *pre, int W) {
+    int c = 0;
+    while (c < W - 1 && i >= pre[c + 1]) c++;
+    return c;
+}
+
+template <bool F64>
+__global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *__restrict__ slab, uint64_t n,
+                                                      float4 *__restrict__ out, float cellsize, double invcpd) {
+    const int s = blockIdx.x;
+    const int W = pl.W;
+    const nf_sink_plan p = pl.sink_plan[s];
+    int pre[NF_MAX_WIDTH + 1];
+    pre[0] = 0;
+    for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
+    const int count = pre[W];
+    const int padded = (count + kWave - 1) / kWave * kWave;
+    const int64_t o = pl.sink_set_start[s];
+    for (int i = threadIdx.x; i < padded; i += blockDim.x) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < count) {
+            const int c = cell_of(i, pre, W);
+            const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
+            v.x = slab[idx];
+            v.y = slab[n + idx];
+            float z = slab[2 * n + idx];
+            const int dc = c - pl.R;
+            if (F64) {
+                z = (float)((double)z + (double)dc * invcpd);
+            } else {  // PencilPlan.cc:43-50: dz = (c-R)*cellsize in FLOAT, added if nonzero
+                const float dz = (float)dc * cellsize;
+                if (dz != 0.f) z = z + dz;
+            }
+            v.z = z;
+        }
+        out[o + i] = v;
+    }
+}
+
+__device__ __forceinline__ const float *slab_ptr(const DevSlabs &sl, int c) {
+    return c == 0 ? sl.pos[0] : c == 1 ? sl.pos[1] : c == 2 ? sl.pos[2] : c == 3 ? sl.pos[3] : sl.pos[4];
+}
+__device__ __forceinline__ uint64_t slab_n(const DevSlabs &sl, int c) {
+    return c == 0 ? sl.n[0] : c == 1 ? sl.n[1] : c == 2 ? sl.n[2] : c == 3 ? sl.n[3] : sl.n[4];
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// One workgroup per source pencil; each wave writes whole 64-source blocks and
+// their bounding boxes (padding excluded).  box[2b] = min, box[2b+1] = max.
+template <bool F64>
+__global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
+                                                        float4 *__restrict__ box, float cellsize,
+                                                        double invcpd) {
+    const int s = blockIdx.x;
+    const int W = pl.W;
+    const nf_source_plan p = pl.src_plan[s];
+    int pre[NF_MAX_WIDTH + 1];
+    pre[0] = 0;
+    for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
+    const int count = pre[W];
+    const int nblk = (count + kWave - 1) / kWave;
+    const int64_t o = pl.src_set_start[s];
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int b = threadIdx.x / kWave; b < nblk; b += blockDim.x / kWave) {
+        const int i = b * kWave + lane;
+        float4 v = make_float4(kFar, 0.f, 0.f, 0.f);
+        const bool real = i < count;
+        if (real) {
+            const int c = cell_of(i, pre, W);
+            const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
+            const float *P = slab_ptr(sl, c);
+            const uint64_t m = slab_n(sl, c);
+            float x = P[idx];
+            const int dc = c - pl.R;
+            if (F64) {
+                x = (float)((double)x + (double)dc * invcpd);
+            } else {  // PencilPlan.cc:226-238
+                const float dx = (float)dc * cellsize;
+                if (dx != 0.f) x = x + dx;
+            }
+            v = make_float4(x, P[m + idx], P[2 * m + idx], 0.f);
+        }
+        out[o + i] = v;
+        const float inf = __builtin_huge_valf();
+        const float mnx = wave_min(real ? v.x : inf), mny = wave_min(real ? v.y : inf), mnz = wave_min(real ? v.z : inf);
+        const float mxx = wave_max(real ? v.x : -inf), mxy = wave_max(real ? v.y : -inf), mxz = wave_max(real ? v.z : -inf);
+        if (lane == 0) {
+            const int64_t gb = o / kWave + b;
+            box[2 * gb] = make_float4(mnx, mny, mnz, 0.f);
+            box[2 * gb + 1] = make_float4(mxx, mxy, mxz, 0.f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain path: the reference's structure and pair forms
+// ---------------------------------------------------------------------------
+
+// DirectGPUKernels.cuh direct_density<0>: single spline :68-108, Plummer
+// :144-174, cubic spline :20-66, cubic Plummer :110-143.  rsqrt is the
+// hardware v_rsq_f32 (the reference uses its own platform's fast rsqrtf).
+template <int SOFT>
+__device__ __forceinline__ void ref_pair(float sx, float sy, float sz, float kx, float ky, float kz,
+                                         const PairConst &pc, float &ax, float &ay, float &az, float &aw) {
+    const float dx = sx - kx, dy = sy - ky, dz = sz - kz;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < pc.b2) aw = aw + pc.b2 - r2;
+    float f;
+    if constexpr (SOFT == NF_SOFT_SINGLE_SPLINE) {
+        const float q = r2 * pc.eps + 1e-32f;
+        f = rsq(q);
+        f = (q > 1.0f) ? f * f * f : (-15.0f * f + 6.0f) * q + 10.0f;
+    } else if constexpr (SOFT == NF_SOFT_PLUMMER) {
+        const float r = rsq(r2 + pc.eps);
+        f = r * r * r;
+    } else if constexpr (SOFT == NF_SOFT_CUBIC_SPLINE) {
+        const float rinv = rsq(r2);
+        const float u = pc.eps * __builtin_amdgcn_sqrtf(r2);  // eps_inv / rinv
+        if (u >= 2.0f) {
+            f = rinv * rinv * rinv;
+        } else if (u >= 1.0f) {
+            const float u3 = u * u * u;
+            f = rinv * rinv * rinv *
+                ((float)(-1.0 / 15.0) + (float)(8.0 / 3.0) * u3 - 3.0f * u3 * u + (float)(6.0 / 5.0) * u3 * u * u -
+                 (float)(1.0 / 6.0) * u3 * u3);
+        } else {
+            f = pc.eps * pc.eps * pc.eps * ((float)(4.0 / 3.0) - (float)(6.0 / 5.0) * u * u + 0.5f * u * u * u);
+        }
+    } else {  // cubic Plummer
+        float r = r2 + 1e-16f;
+        r *= r * rsq(r);
+        r += pc.eps;
+        f = __builtin_amdgcn_rcpf(r);
+    }
+    ax -= f * dx;
+    ay -= f * dy;
+    az -= f * dz;
+}
+
+template <int SOFT>
+__global__ __launch_bounds__(64) void k_plain(DevPlan pl, PairConst pc, const float4 *__restrict__ sinks,
+                                              const float4 *__restrict__ src, float4 *__restrict__ out) {
+    const int blk = blockIdx.x;
+    const int s = pl.parent[blk];
+    const int id = blk * kWave + threadIdx.x;
+    const float4 k = sinks[id];  // padded ids hold zeros (PencilKernel.cu:109-117)
+    float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+    const int W = pl.W;
+    for (int c = s * W; c < s * W + W; c++) {
+        const int si = pl.ilist[c];
+        const float yo = pl.yoff[c];
+        const int st = pl.src_set_start[si], n = pl.src_set_count[si];
+        const float4 *sp = src + st;
+        const int nB = n / kWave;
+        for (int b = 0; b < nB; b++) {  // FullDirectTile with BLOCK_ACCUM
+            float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll 16
+            for (int i = 0; i < kWave; i++) {
+                const float4 q = sp[b * kWave + i];
+                ref_pair<SOFT>(q.x, q.y + yo, q.z, k.x, k.y, k.z, pc, tx, ty, tz, aw);
+            }
+            ax += tx;
+            ay += ty;
+            az += tz;
+        }
+        for (int i = nB * kWave; i < n; i++) {  // PartialDirectTile
+            const float4 q = sp[i];
+            ref_pair<SOFT>(q.x, q.y + yo, q.z, k.x, k.y, k.z, pc, ax, ay, az, aw);
+        }
+    }
+    if ((uint32_t)id < pl.sink_idmax[s]) out[id] = make_float4(ax, ay, az, aw);
+}
+
+// Addresses of the W partials of the sinks of cell (j_low + j, z): the cell is
+// at position c of pencil k = z - c (mod cpd).  Returns the cell's slab start
+// and count; base[c] is the padded slot of its first particle in pencil k.
+__device__ __forceinline__ void partial_slots(const DevPlan &pl, int j, int z, int64_t *base, uint32_t *cstart,
+                                              int *cn) {
+    const int Nk = pl.Nk, W = pl.W, cpd = pl.cpd;
+    const nf_sink_plan &own = pl.sink_plan[j * Nk + z];  // pencil k = z starts with cell z
+    *cstart = own.cell[0].start;
+    *cn = own.cell[0].N;
+    for (int c = 0; c < W; c++) {
+        int k = z - c;
+        if (k < 0) k += cpd;
+        const int s = j * Nk + k;
+        const nf_sink_plan &pk = pl.sink_plan[s];
+        int off = 0;
+        for (int c2 = 0; c2 < c; c2++) off += pk.cell[c2].N;
+        base[c] = (int64_t)pl.sink_set_start[s] + off;
+    }
+}
+
+// FP32 co-add in the reference's order: the pencils of a skewer are applied
+// in increasing k, the first one setting the value and the rest adding.  For
+// cell z that is c = min(z, W-1) .. 0, then W-1 .. z+1.
+__global__ __launch_bounds__(64) void k_coadd_plain(DevPlan pl, const float4 *__restrict__ part,
+                                                    float4 *__restrict__ acc) {
+    const int z = blockIdx.x, j = blockIdx.y;
+    int64_t base[NF_MAX_WIDTH];
+    uint32_t cstart;
+    int cn;
+    partial_slots(pl, j, z, base, &cstart, &cn);
+    const int W = pl.W;
+    int order[NF_MAX_WIDTH], no = 0;
+    for (int c = (z < W - 1 ? z : W - 1); c >= 0; c--) order[no++] = c;
+    for (int c = W - 1; c > z; c--) order[no++] = c;
+    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
+        float4 a = part[base[order[0]] + i];
+        for (int t = 1; t < W; t++) {
+            const float4 p = part[base[order[t]] + i];
+            a.x += p.x;
+            a.y += p.y;
+            a.z += p.z;
+            a.w += p.w;
+        }
+        acc[cstart + i] = a;
+    }
+}
+
+// FP64 co-add: sum of the W double partials, scaled back to the reference's
+// units (out_scale applies to a, not to w).
+__global__ __launch_bounds__(64) void k_coadd_f64(DevPlan pl, const double *__restrict__ part, double out_scale,
+                                                  float4 *__restrict__ acc) {
+    const int z = blockIdx.x, j = blockIdx.y;
+    int64_t base[NF_MAX_WIDTH];
+    uint32_t cstart;
+    int cn;
+    partial_slots(pl, j, z, base, &cstart, &cn);
+    const int W = pl.W;
+    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        for (int c = 0; c < W; c++) {
+            const double *p = part + 4 * (base[c] + i);
+            a0 += p[0];
+            a1 += p[1];
+            a2 += p[2];
+            a3 += p[3];
+        }
+        acc[cstart + i] = make_float4((float)(a0 * out_scale), (float)(a1 * out_scale), (float)(a2 * out_scale),
+                                      (float)a3);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched path
+// ---------------------------------------------------------------------------
+
+// Pair kernel in box units.  r2 carries pc.r2add; for the single spline the
+// result is g/e^3 (the e^3 is restored in the co-add).  FAST evaluates the
+// unsoftened branch only and is valid when no pair of the tile is softened.
+template <int SOFT, bool FAST>
+__device__ __forceinline__ void pair_box(float sx, float sy, float sz, float kx, float ky, float kz,
+                                         const PairConst &pc, float &ax, float &ay, float &az, float &aw) {
+    const float dx = sx - kx, dy = sy - ky, dz = sz - kz;
+    float r2 = fmaf(dx, dx, pc.r2add);
+    r2 = fmaf(dy, dy, r2);
+    r2 = fmaf(dz, dz, r2);
+    aw += clamp01(pc.b2x - r2);
+    float g;
+    if constexpr (SOFT == NF_SOFT_SINGLE_SPLINE) {
+        const float f = rsq(r2);
+        const float f3 = f * f * f;
+        if constexpr (FAST) g = f3;
+        else g = fminf(f3, fmaf(fmaf(pc.c1, f, pc.c2), r2, pc.c3));
+    } else if constexpr (SOFT == NF_SOFT_PLUMMER) {
+        const float f = rsq(r2);
+        g = f * f * f;
+    } else if constexpr (SOFT == NF_SOFT_CUBIC_SPLINE) {
+        const float rinv = rsq(r2);
+        const float u = pc.eps * __builtin_amdgcn_sqrtf(r2);
+        if (u >= 2.0f) {
+            g = rinv * rinv * rinv;
+        } else if (u >= 1.0f) {
+            const float u3 = u * u * u;
+            g = rinv * rinv * rinv *
+                ((float)(-1.0 / 15.0) + (float)(8.0 / 3.0) * u3 - 3.0f * u3 * u + (float)(6.0 / 5.0) * u3 * u * u -
+                 (float)(1.0 / 6.0) * u3 * u3);
+        } else {
+            g = pc.eps * pc.eps * pc.eps * ((float)(4.0 / 3.0) - (float)(6.0 / 5.0) * u * u + 0.5f * u * u * u);
+        }
+    } else {  // cubic Plummer; r2 already holds + TAU2.  r^3 as r2*sqrt(r2) so a
+              // padding source (r2 = inf) gives 1/inf = 0, not inf*0.
+        const float r = r2 * __builtin_amdgcn_sqrtf(r2) + pc.eps;
+        g = __builtin_amdgcn_rcpf(r);
+    }
+    ax = fmaf(-g, dx, ax);
+    ay = fmaf(-g, dy, ay);
+    az = fmaf(-g, dz, az);
+}
+
+// TwoSum (Knuth): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void merge(float &h, float &l, float t) {
+    float s, e;
+    two_sum(h, t, s, e);
+    h = s;
+    l += e;
+}
+
+// Squared gap between two axis-aligned boxes (0 if they overlap).
+__device__ __forceinline__ float box_gap2(float alx, float aly, float alz, float ahx, float ahy, float ahz, float4 bl,
+                                          float4 bh) {
+    const float gx = fmaxf(0.f, fmaxf(bl.x - ahx, alx - bh.x));
+    const float gy = fmaxf(0.f, fmaxf(bl.y - ahy, aly - bh.y));
+    const float gz = fmaxf(0.f, fmaxf(bl.z - ahz, alz - bh.z));
+    return gx * gx + gy * gy + gz * gz;
+}
+
+struct BatchCtx {
+    DevPlan pl;
+    PairConst pc;
+    double invcpd;
+    float soft2;  // squared distance above which the unsoftened branch is exact (with margin)
+};
+
+// Register tile: 64*S sinks of one pencil on the lanes; sources are wave
+// uniform and come through the scalar cache.
+template <int SOFT, int S>
+__device__ __forceinline__ void full_item(const BatchCtx &cx, const WorkItem &w, int lane,
+                                          const float4 *__restrict__ sinks, const float4 *__restrict__ src,
+                                          const float4 *__restrict__ box, double *__restrict__ out) {
+    const DevPlan &pl = cx.pl;
+    const int s = w.sinkset;
+    const int base = pl.sink_set_start[s] + w.first;
+    float kx[S], ky[S], kz[S];
+    float blx = __builtin_huge_valf(), bly = blx, blz = blx, bhx = -blx, bhy = -blx, bhz = -blx;
+#pragma unroll
+    for (int u = 0; u < S; u++) {
+        const float4 k = sinks[base + lane + kWave * u];
+        kx[u] = k.x;
+        ky[u] = k.y;
+        kz[u] = k.z;
+        blx = fminf(blx, k.x);
+        bly = fminf(bly, k.y);
+        blz = fminf(blz, k.z);
+        bhx = fmaxf(bhx, k.x);
+        bhy = fmaxf(bhy, k.y);
+        bhz = fmaxf(bhz, k.z);
+    }
+    blx = wave_min(blx);
+    bly = wave_min(bly);
+    blz = wave_min(blz);
+    bhx = wave_max(bhx);
+    bhy = wave_max(bhy);
+    bhz = wave_max(bhz);
+
+    float hx[S], hy[S], hz[S], lx[S], ly[S], lz[S], hw[S];
+#pragma unroll
+    for (int u = 0; u < S; u++) hx[u] = hy[u] = hz[u] = lx[u] = ly[u] = lz[u] = hw[u] = 0.f;
+
+    const int W = pl.W;
+    for (int y = 0; y < W; y++) {
+        const int si = pl.ilist[s * W + y];
+        const int st = pl.src_set_start[si];
+        const int n = pl.src_set_count[si];
+        // Source y + (y-R)/cpd == sink y - (y-R)/cpd on the other side; the
+        // cell-centre offset is formed in FP64.
+        const double yo = (double)(y - pl.R) * cx.invcpd;
+        float kyy[S];
+#pragma unroll
+        for (int u = 0; u < S; u++) kyy[u] = (float)((double)ky[u] - yo);
+        const float sly = (float)((double)bly - yo), shy = (float)((double)bhy - yo);
+        const int nblk = (n + kWave - 1) / kWave;
+        for (int b = 0; b < nblk; b++) {
+            const float4 *sp = src + st + b * kWave;
+            const int cnt8 = (min(kWave, n - b * kWave) + 7) & ~7;
+            const int64_t gb = (int64_t)st / kWave + b;
+            const bool fast = (SOFT == NF_SOFT_PLUMMER) ||
+                              (SOFT == NF_SOFT_SINGLE_SPLINE && cx.pc.fast_ok &&
+                               box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]) > cx.soft2);
+            float tx[S], ty[S], tz[S], tw[S];
+#pragma unroll
+            for (int u = 0; u < S; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
+            if (fast) {
+                for (int i = 0; i < cnt8; i += 8) {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float4 sv = sp[i + q];
+#pragma unroll
+                        for (int u = 0; u < S; u++)
+                            pair_box<SOFT, true>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
+                                                 tw[u]);
+                    }
+                }
+            } else {
+                for (int i = 0; i < cnt8; i += 8) {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float4 sv = sp[i + q];
+#pragma unroll
+                        for (int u = 0; u < S; u++)
+                            pair_box<SOFT, false>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
+                                                  tw[u]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < S; u++) {
+                merge(hx[u], lx[u], tx[u]);
+                merge(hy[u], ly[u], ty[u]);
+                merge(hz[u], lz[u], tz[u]);
+                hw[u] += tw[u];
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < S; u++) {
+        double *o = out + 4 * (int64_t)(base + lane + kWave * u);
+        o[0] = (double)hx[u] + (double)lx[u];
+        o[1] = (double)hy[u] + (double)ly[u];
+        o[2] = (double)hz[u] + (double)lz[u];
+        o[3] = (double)hw[u];
+    }
+}
+
+__device__ __forceinline__ void wave_two_sum(float &h, float &l) {
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float h2 = __shfl_xor(h, off);
+        const float l2 = __shfl_xor(l, off);
+        float s, e;
+        two_sum(h, h2, s, e);
+        h = s;
+        l = l + l2 + e;
+    }
+}
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Remainder of a pencil (fewer than 64*S sinks): sources on the lanes, SU
+// sinks wave-uniform per pass, per-lane partial sums reduced across the wave.
+template <int SOFT, int SU>
+__device__ __forceinline__ void rem_item(const BatchCtx &cx, const WorkItem &w, int lane,
+                                         const float4 *__restrict__ sinks, const float4 *__restrict__ src,
+                                         const float4 *__restrict__ box, double *__restrict__ out) {
+    constexpr int TC = 16;  // chunks of 64 sources per TwoSum tile
+    const DevPlan &pl = cx.pl;
+    const int s = w.sinkset;
+    const int base = pl.sink_set_start[s] + w.first;
+    const int r = w.nsink;
+    const int W = pl.W;
+    for (int g = 0; g < r; g += SU) {
+        float kx[SU], ky[SU], kz[SU];
+        float blx = __builtin_huge_valf(), bly = blx, blz = blx, bhx = -blx, bhy = -blx, bhz = -blx;
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int gi = g + u < r ? g + u : r - 1;
+            const float4 k = sinks[base + gi];
+            kx[u] = k.x;
+            ky[u] = k.y;
+            kz[u] = k.z;
+            blx = fminf(blx, k.x);
+            bly = fminf(bly, k.y);
+            blz = fminf(blz, k.z);
+            bhx = fmaxf(bhx, k.x);
+            bhy = fmaxf(bhy, k.y);
+            bhz = fmaxf(bhz, k.z);
+        }
+        float hx[SU], hy[SU], hz[SU], lx[SU], ly[SU], lz[SU], hw[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) hx[u] = hy[u] = hz[u] = lx[u] = ly[u] = lz[u] = hw[u] = 0.f;
+        for (int y = 0; y < W; y++) {
+            const int si = pl.ilist[s * W + y];
+            const int st = pl.src_set_start[si];
+            const int n = pl.src_set_count[si];
+            const double yo = (double)(y - pl.R) * cx.invcpd;
+            float kyy[SU];
+#pragma unroll
+            for (int u = 0; u < SU; u++) kyy[u] = (float)((double)ky[u] - yo);
+            const float sly = (float)((double)bly - yo), shy = (float)((double)bhy - yo);
+            const int nblk = (n + kWave - 1) / kWave;
+            for (int b0 = 0; b0 < nblk; b0 += TC) {
+                const int b1 = min(nblk, b0 + TC);
+                float tx[SU], ty[SU], tz[SU], tw[SU];
+#pragma unroll
+                for (int u = 0; u < SU; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
+                for (int b = b0; b < b1; b++) {
+                    const float4 sv = src[st + b * kWave + lane];
+                    const int64_t gb = (int64_t)st / kWave + b;
+                    const bool fast = (SOFT == NF_SOFT_PLUMMER) ||
+                                      (SOFT == NF_SOFT_SINGLE_SPLINE && cx.pc.fast_ok &&
+                                       box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]) > cx.soft2);
+                    if (fast) {
+#pragma unroll
+                        for (int u = 0; u < SU; u++)
+                            pair_box<SOFT, true>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
+                                                 tw[u]);
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < SU; u++)
+                            pair_box<SOFT, false>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
+                                                  tw[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < SU; u++) {
+                    merge(hx[u], lx[u], tx[u]);
+                    merge(hy[u], ly[u], ty[u]);
+                    merge(hz[u], lz[u], tz[u]);
+                    hw[u] += tw[u];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            wave_two_sum(hx[u], lx[u]);
+            wave_two_sum(hy[u], ly[u]);
+            wave_two_sum(hz[u], lz[u]);
+            const float ws = wave_sum(hw[u]);
+            if (lane == 0 && g + u < r) {
+                double *o = out + 4 * (int64_t)(base + g + u);
+                o[0] = (double)hx[u] + (double)lx[u];
+                o[1] = (double)hy[u] + (double)ly[u];
+                o[2] = (double)hz[u] + (double)lz[u];
+                o[3] = (double)ws;
+            }
+        }
+    }
+}
+
+constexpr int kRemSinks = 4;
+
+// Persistent kernel: every wave pops items until the queue is empty.
+template <int SOFT, int S>
+__global__ __launch_bounds__(256) void k_batched(BatchCtx cx, const float4 *__restrict__ sinks,
+                                                 const float4 *__restrict__ src, const float4 *__restrict__ box,
+                                                 const WorkItem *__restrict__ items, int32_t nitems,
+                                                 int32_t *__restrict__ counter, double *__restrict__ out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(counter, 1);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= nitems) return;
+        const WorkItem w = items[it];
+        if (w.kind == 0) full_item<SOFT, S>(cx, w, lane, sinks, src, box, out);
+        else rem_item<SOFT, kRemSinks>(cx, w, lane, sinks, src, box, out);
+    }
+}
+
+__global__ void k_zero_counter(int32_t *c) { *c = 0; }
+
+template <int SOFT>
+int launch_plain_t(const DevPlan &pl, const PairConst &pc, const void *sinks4, const void *sources4, void *partial4,
+                   hipStream_t st) {
+    if (pl.nsink_blocks == 0) return 0;
+    hipLaunchKernelGGL(k_plain<SOFT>, dim3(pl.nsink_blocks), dim3(kWave), 0, st, pl, pc,
+                       static_cast<const float4 *>(sinks4), static_cast<const float4 *>(sources4),
+                       static_cast<float4 *>(partial4));
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+static float4 *box_of(void *sources4, int32_t nsrc_blocks) {
+    // Bounding boxes follow the padded source stream in the same allocation.
+    return static_cast<float4 *>(sources4) + (int64_t)nsrc_blocks * kWave;
+}
+
+int device_cu_count() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return n;
+}
+
+int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void *sinks4, void *sources4,
+                  void *stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const float cellsize = (float)(1.0 / (double)pl.cpd);
+    const double invcpd = 1.0 / (double)pl.cpd;
+    float4 *box = box_of(sources4, pl.nsrc_blocks);
+    if (pl.nsink_sets > 0) {
+        if (fp64_offsets)
+            hipLaunchKernelGGL(k_gather_sinks<true>, dim3(pl.nsink_sets), dim3(256), 0, st, pl, sl.pos[pl.R],
+                               sl.n[pl.R], static_cast<float4 *>(sinks4), cellsize, invcpd);
+        else
+            hipLaunchKernelGGL(k_gather_sinks<false>, dim3(pl.nsink_sets), dim3(256), 0, st, pl, sl.pos[pl.R],
+                               sl.n[pl.R], static_cast<float4 *>(sinks4), cellsize, invcpd);
+    }
+    if (pl.nsrc_sets > 0) {
+        if (fp64_offsets)
+            hipLaunchKernelGGL(k_gather_sources<true>, dim3(pl.nsrc_sets), dim3(256), 0, st, pl, sl,
+                               static_cast<float4 *>(sources4), box, cellsize, invcpd);
+        else
+            hipLaunchKernelGGL(k_gather_sources<false>, dim3(pl.nsrc_sets), dim3(256), 0, st, pl, sl,
+                               static_cast<float4 *>(sources4), box, cellsize, invcpd);
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const void *sinks4, const void *sources4,
+                 void *partial4, void *stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (softening) {
+        case NF_SOFT_PLUMMER: return launch_plain_t<NF_SOFT_PLUMMER>(pl, pc, sinks4, sources4, partial4, st);
+        case NF_SOFT_SINGLE_SPLINE:
+            return launch_plain_t<NF_SOFT_SINGLE_SPLINE>(pl, pc, sinks4, sources4, partial4, st);
+        case NF_SOFT_CUBIC_SPLINE:
+            return launch_plain_t<NF_SOFT_CUBIC_SPLINE>(pl, pc, sinks4, sources4, partial4, st);
+        case NF_SOFT_CUBIC_PLUMMER:
+            return launch_plain_t<NF_SOFT_CUBIC_PLUMMER>(pl, pc, sinks4, sources4, partial4, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
+int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, void *stream) {
+    if (pl.j_width == 0) return 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_coadd_plain, dim3(pl.cpd, pl.j_width), dim3(kWave), 0, st, pl,
+                       static_cast<const float4 *>(partial4), reinterpret_cast<float4 *>(acc4));
+    return (int)hipGetLastError();
+}
+
+int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4, void *stream) {
+    if (pl.j_width == 0) return 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_coadd_f64, dim3(pl.cpd, pl.j_width), dim3(kWave), 0, st, pl,
+                       static_cast<const double *>(partial_d4), out_scale, reinterpret_cast<float4 *>(acc4));
+    return (int)hipGetLastError();
+}
+
+template <int SOFT, int S>
+static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const void *sinks4, const void *sources4,
+                            const WorkItem *items, int32_t nitems, int32_t *counter, void *partial_d4,
+                            hipStream_t st) {
+    const float4 *box = box_of(const_cast<void *>(sources4), cx.pl.nsrc_blocks);
+    int blocks_per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_batched<SOFT, S>, 256, 0) != hipSuccess ||
+        blocks_per_cu <= 0)
+        blocks_per_cu = 1;
+    if (ts.waves_per_cu > 0) blocks_per_cu = std::max(1, std::min(blocks_per_cu, ts.waves_per_cu / 4));
+    const int ncu = device_cu_count();
+    int grid = std::max(1, ncu) * blocks_per_cu;
+    const int need = (nitems + 3) / 4;
+    if (grid > need) grid = std::max(1, need);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL((k_batched<SOFT, S>), dim3(grid), dim3(256), 0, st, cx, static_cast<const float4 *>(sinks4),
+                       static_cast<const float4 *>(sources4), box, items, nitems, counter,
+                       static_cast<double *>(partial_d4));
+    return (int)hipGetLastError();
+}
+
+template <int SOFT>
+static int launch_batched_soft(const BatchCtx &cx, const TileShape &ts, const void *sinks4, const void *sources4,
+                               const WorkItem *items, int32_t nitems, int32_t *counter, void *partial_d4,
+                               hipStream_t st) {
+    switch (ts.sinks_per_lane) {
+        case 1: return launch_batched_s<SOFT, 1>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
+        case 2: return launch_batched_s<SOFT, 2>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
+        case 4: return launch_batched_s<SOFT, 4>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
+int launch_batched(const DevPlan &pl, int softening, const PairConst &pc, const TileShape &ts, const void *sinks4,
+                   const void *sources4, const WorkItem *items, int32_t nitems, int32_t *counter, void *partial_d4,
+                   void *stream) {
+    if (nitems == 0) return 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    BatchCtx cx;
+    cx.pl = pl;
+    cx.pc = pc;
+    cx.invcpd = 1.0 / (double)pl.cpd;
+    // The unsoftened branch is exact for r^2 > e^2 = 1/eps; the box test uses
+    // a relative margin far above FP32 rounding of the box corners.
+    cx.soft2 = (softening == NF_SOFT_SINGLE_SPLINE) ? (float)((1.0 / (double)pc.eps) * 1.01) : 0.f;
+    switch (softening) {
+        case NF_SOFT_PLUMMER:
+            return launch_batched_soft<NF_SOFT_PLUMMER>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
+        case NF_SOFT_SINGLE_SPLINE:
+            return launch_batched_soft<NF_SOFT_SINGLE_SPLINE>(cx, ts, sinks4, sources4, items, nitems, counter,
+                                                              partial_d4, st);
+        case NF_SOFT_CUBIC_SPLINE:
+            return launch_batched_soft<NF_SOFT_CUBIC_SPLINE>(cx, ts, sinks4, sources4, items, nitems, counter,
+                                                             partial_d4, st);
+        case NF_SOFT_CUBIC_PLUMMER:
+            return launch_batched_soft<NF_SOFT_CUBIC_PLUMMER>(cx, ts, sinks4, sources4, items, nitems, counter,
+                                                              partial_d4, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
+}  // namespace nf

@@ -1,0 +1,86 @@
+// Internal declarations shared by the host runtime (runtime.cpp), the
+// planner (planner.cpp) and the device kernels (kernels.hip).
+#ifndef NF_INTERNAL_H
+#define NF_INTERNAL_H
+
+#include <cstdint>
+#include <string>
+
+#include "nf_direct.h"
+
+namespace nf {
+
+// Record an error message for nf_last_error() on this thread; returns code.
+int fail(int code, const std::string &msg);
+
+// Work item of the batched kernel: a run of `nsink` sinks starting at
+// `first` within sink pencil `sinkset`.  kind 0: register tile of 64*S sinks,
+// sinks on lanes; kind 1: remainder (< 64*S sinks), sources on lanes.
+struct WorkItem {
+    int32_t sinkset;
+    int32_t first;
+    int32_t nsink;
+    int32_t kind;
+};
+
+// Device views of the plan arrays of one SIC.
+struct DevPlan {
+    const int32_t *sink_set_start;
+    const int32_t *sink_set_count;
+    const uint32_t *sink_idmax;
+    const nf_sink_plan *sink_plan;
+    const int32_t *src_set_start;
+    const int32_t *src_set_count;
+    const nf_source_plan *src_plan;
+    const int32_t *parent;
+    const int32_t *ilist;
+    const float *yoff;
+    int32_t nsink_sets, nsrc_sets, nsink_blocks, nsrc_blocks;
+    int32_t W, R, cpd, Nk, j_width, j_low;
+};
+
+// The slabs of one SIC on the device.
+struct DevSlabs {
+    const float *pos[NF_MAX_WIDTH];  // PosXYZ slab of SlabId-R+c
+    uint64_t n[NF_MAX_WIDTH];
+};
+
+// Constants of the pair kernel, derived on the host from (softening, eps, b2).
+struct PairConst {
+    float eps;        // the reference's eps parameter
+    float b2;         // FOF density radius^2 as given
+    float r2add;      // added to r^2 before rsqrt: tau (spline forms) or e^2 (Plummer)
+    float b2x;        // density threshold against that r^2: b2 + r2add
+    float c1, c2, c3; // single spline, box-unit polynomial (-15 e^-4, 6 e^-5, 10 e^-3)
+    float theta;      // tile density sum below which no pair is softened (< 0: none)
+    double out_scale; // e^3 for the box-unit spline, 1 otherwise
+    int32_t fast_ok;  // 1 when the unsoftened fast path may be used
+};
+
+int derive_pair_const(int softening, float eps, float b2, PairConst *pc);
+
+// Batched-path tuning knobs.
+struct TileShape {
+    int sinks_per_lane;   // S: 1, 2 or 4
+    int tile;             // TwoSum tile depth in sources (sinks on lanes)
+    int rem_sinks;        // uniform sinks per pass in the remainder path
+    int waves_per_cu;     // persistent grid size (0: from occupancy)
+};
+
+// Kernel launchers (kernels.hip).  All enqueue on `stream` and return a HIP
+// error code (0 on success).
+int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void *sinks4,
+                  void *sources4, void *stream);
+int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const void *sinks4,
+                 const void *sources4, void *partial4, void *stream);
+int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, void *stream);
+int launch_batched(const DevPlan &pl, int softening, const PairConst &pc, const TileShape &ts,
+                   const void *sinks4, const void *sources4, const WorkItem *items, int32_t nitems,
+                   int32_t *counter, void *partial_d4, void *stream);
+int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
+                     void *stream);
+int device_cu_count();
+
+}  // namespace nf
+
+#endif  // NF_INTERNAL_H

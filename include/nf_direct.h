@@ -1,0 +1,214 @@
+/*
+ * nf_direct.h -- C ABI of the MI355X near-field direct force module.
+ *
+ * This is the drop-in boundary that replaces Abacus's CUDA direct module
+ * (singlestep/Direct/gpudirect.a, exported API in DeviceFunctions.h:8-26 and
+ * SetInteractionCollection::GPUExecute).  A SetInteractionCollection (SIC) is
+ * handed over as a plain-old-data descriptor (nf_sic) whose arrays are the
+ * SIC's own members; the module computes the pencil-on-pencil directs on the
+ * GPU and fills the sink slab's accelerations (float4: ax, ay, az, FOF
+ * density), exactly as the reference's GPU thread does after its co-add.
+ *
+ * All entry points return NF_OK (0) or a negative nf_status; they never abort.
+ * nf_last_error() describes the most recent failure of the calling thread.
+ * No torch or C++ types cross this boundary.
+ */
+#ifndef NF_DIRECT_H
+#define NF_DIRECT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compile-time maxima, as configure.ac:155-162 fixes NFRADIUS and NFBlockSize. */
+#define NF_MAX_RADIUS 2
+#define NF_MAX_WIDTH (2 * NF_MAX_RADIUS + 1)
+#define NF_BLOCK_SIZE 64
+
+typedef enum nf_status {
+    NF_OK = 0,
+    NF_ERR_INVALID = -1,   /* bad argument or inconsistent descriptor */
+    NF_ERR_HIP = -2,       /* a HIP runtime call failed */
+    NF_ERR_CAPACITY = -3,  /* task exceeds the buffer capacity given to nf_setup */
+    NF_ERR_STATE = -4,     /* nf_setup not called, or called twice */
+    NF_ERR_NODEV = -5,     /* no usable GPU */
+    NF_ERR_NOMEM = -6      /* host or device allocation failed */
+} nf_status;
+
+/* Softening forms (configure.ac:315-328; kernels DirectGPUKernels.cuh). The
+ * meaning of nf_sic.eps follows NearFieldDriver's ctor (directdriver.cpp:105-117):
+ * PLUMMER eps = e^2, SINGLE_SPLINE eps = 1/e^2, CUBIC_SPLINE eps = 1/e,
+ * CUBIC_PLUMMER eps = e^3, with e the internal softening length. */
+typedef enum nf_softening {
+    NF_SOFT_PLUMMER = 0,
+    NF_SOFT_SINGLE_SPLINE = 1,
+    NF_SOFT_CUBIC_SPLINE = 2,
+    NF_SOFT_CUBIC_PLUMMER = 3
+} nf_softening;
+
+/* Execution path.  PLAIN: one launch per SIC over 64-sink blocks with FP32
+ * tile accumulation and an FP32 k-ordered co-add (the reference's structure,
+ * PencilKernel.cu:97-189 + PencilPlan.cc:140-207).  BATCHED: one persistent
+ * launch per SIC draining a work queue sorted by interaction count, TwoSum
+ * compensated FP32 accumulation, FP64 co-add of the per-pencil partials. */
+typedef enum nf_path { NF_PATH_PLAIN = 0, NF_PATH_BATCHED = 1 } nf_path;
+
+/* Where the slab pointers of an nf_sic live. */
+typedef enum nf_mem { NF_MEM_HOST = 0, NF_MEM_DEVICE = 1 } nf_mem;
+
+/* CellPencilPlan (SetInteractionCollection.hh:76-87): start index of a cell
+ * in its PosXYZ slab and its particle count.  Same layout as the reference. */
+typedef struct nf_cell_plan {
+    uint32_t start;
+    int32_t N;
+} nf_cell_plan;
+
+/* SinkPencilPlan (SetInteractionCollection.hh:94-115), non-PARALLEL layout. */
+typedef struct nf_sink_plan {
+    nf_cell_plan cell[NF_MAX_WIDTH];
+    int32_t wrapcell;
+} nf_sink_plan;
+
+/* SourcePencilPlan (SetInteractionCollection.hh:121-132). */
+typedef struct nf_source_plan {
+    nf_cell_plan cell[NF_MAX_WIDTH];
+} nf_source_plan;
+
+/* cellinfo (particlestruct.cpp:334-392), same layout as the reference. */
+typedef struct nf_cellinfo {
+    uint64_t startindex;
+    uint64_t startindex_with_ghost;
+    int32_t count;
+    int32_t active;
+    float mean_square_velocity;
+    float max_component_velocity;
+    float max_component_acceleration;
+} nf_cellinfo;
+
+/* The SIC descriptor: the members of SetInteractionCollection
+ * (SetInteractionCollection.hh:171-283) that the GPU module reads. */
+typedef struct nf_sic {
+    int32_t SlabId, j_low, j_high, k_low, k_high, Nk, j_width;
+    int32_t cpd, nfradius, nfwidth;
+    float eps;           /* softening parameter, see nf_softening */
+    float b2;            /* FOF density radius squared (0 disables w) */
+    int32_t softening;   /* nf_softening */
+    int32_t mem;         /* nf_mem of the three slab pointer fields below */
+
+    const float *SinkPosSlab;                  /* PosXYZ of slab SlabId: X|Y|Z */
+    const float *SourcePosSlab[NF_MAX_WIDTH];  /* PosXYZ of slabs SlabId-R..SlabId+R */
+    uint64_t Nslab[NF_MAX_WIDTH];              /* particles per slab (with ghosts) */
+    float *SinkAccSlab;                        /* float4 per particle of slab SlabId */
+
+    int32_t NSinkSets;
+    const int32_t *SinkSetStart;
+    const int32_t *SinkSetCount;
+    const uint32_t *SinkSetIdMax;
+    const nf_sink_plan *SinkPlan;
+
+    int32_t NSourceSets;
+    const int32_t *SourceSetStart;
+    const int32_t *SourceSetCount;
+    const nf_source_plan *SourcePlan;
+
+    int32_t NSinkBlocks, NSourceBlocks, InteractionCount;
+    const int32_t *SinkBlockParentPencil;
+    const int32_t *SinkSourceInteractionList;
+    const float *SinkSourceYOffset;
+
+    uint64_t DirectTotal, PaddedDirectTotal;
+    uint64_t SinkTotal, SourceTotal, PaddedSinkTotal, PaddedSourceTotal;
+} nf_sic;
+
+/* Per-task timers and counters, the analogue of the SIC's STimers and byte
+ * counts that NearFieldDriver::Finalize aggregates (directdriver.cpp:453-474). */
+typedef struct nf_task_stats {
+    double device_thread_s;   /* wall time of the task on its worker */
+    double fill_s;            /* plan upload + slab upload (host mode) */
+    double kernel_s;          /* gather + direct kernel + co-add, GPU time */
+    double direct_kernel_s;   /* the direct kernel alone, GPU time */
+    double copyback_s;        /* acceleration copy back (host mode) */
+    uint64_t bytes_to_device, bytes_from_device;
+    uint64_t direct_total, padded_direct_total;
+    int32_t assigned_buffer;
+    int32_t path;
+} nf_task_stats;
+
+typedef struct nf_task nf_task;
+
+/* Setup parameters, the arguments of GPUSetup (GPUControl.cu:357-363). */
+typedef struct nf_setup_params {
+    int32_t cpd;
+    uint64_t max_buffer_bytes;    /* device bytes per buffer (MaxBufferSize) */
+    int32_t n_gpu;                /* number of GPUs to use (1 on this build) */
+    int32_t buffers_per_device;   /* DirectBPD: worker threads/streams per GPU */
+    int32_t use_pinned;           /* UsePinnedGPUMemory: -1 auto, 0 no, 1 yes */
+    int32_t path;                 /* nf_path used by nf_submit */
+    int32_t sinks_per_lane;       /* batched path register tile: 1, 2 or 4 (0: auto) */
+    int32_t source_tile;          /* batched path TwoSum tile depth (0: auto) */
+} nf_setup_params;
+
+/* GetNGPU (GPUControl.cu:267-271). */
+int nf_get_ngpu(int32_t *n);
+
+/* GetDeviceInfo (GPUControl.cu:274-285): min over GPUs of 0.8 * total
+ * memory, and the device name. */
+int nf_get_device_info(uint64_t *bytes, char *name, size_t name_len);
+
+/* GPUSetup (GPUControl.cu:357-482): start buffers_per_device workers per GPU,
+ * each owning one HIP stream and one device buffer, and report the largest
+ * sink/source block counts a SIC may have (MaxSinkBlocks/MaxSourceBlocks). */
+int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int32_t *max_source_blocks);
+
+/* GPUReset (GPUControl.cu:492-510): stop the workers and free all buffers. */
+int nf_reset(void);
+
+/* SetInteractionCollection::GPUExecute (GPUControl.cu:235-261): queue the SIC
+ * on the worker chosen by its y-midpoint (or run it on the caller's thread when
+ * blocking != 0) and return a task handle.  The slabs and the plan arrays must
+ * stay alive until the task completes. */
+int nf_submit(const nf_sic *sic, int32_t blocking, nf_task **task);
+
+/* SetInteractionCollection::CheckCompletion (PencilOnPencil.cc:434-436):
+ * 1 when done, 0 while running, < 0 if the task failed. */
+int nf_check_completion(nf_task *task);
+
+/* Block until the task is done; returns its final status. */
+int nf_wait(nf_task *task);
+
+/* Copy out the task's timers and counters (valid once complete). */
+int nf_get_task_stats(nf_task *task, nf_task_stats *out);
+
+/* Release a completed task handle. */
+int nf_release(nf_task *task);
+
+/* Thread-local description of the last error. */
+const char *nf_last_error(void);
+
+/* ---- planning (host) ------------------------------------------------- */
+
+/* ComputeSICSize (PencilOnPencil.cc:130-150): bytes of arena a SIC planned by
+ * nf_plan_sic needs for sink rows [j_low, j_high), given the particle counts
+ * of the slabs involved (an upper bound, like the reference's). */
+uint64_t nf_sic_arena_bytes(int32_t cpd, int32_t nfradius, int32_t j_low, int32_t j_high,
+                            uint64_t max_slab_particles);
+
+/* SetInteractionCollection ctor (PencilOnPencil.cc:180-418) for the
+ * non-PARALLEL build: plan the SIC of slab `slab`, sink rows [j_low, j_high).
+ * cellinfo[c] is the cellinfo slab (cpd*cpd cells, y-major) of slab
+ * slab-R+c, c = 0..2R; pos_slab[c] its PosXYZ slab and nslab[c] its size.
+ * Arrays are carved from `arena` (arena_bytes long); *out is filled.  The
+ * caller sets out->SinkAccSlab and out->mem afterwards if needed. */
+int nf_plan_sic(int32_t cpd, int32_t nfradius, int32_t slab, int32_t j_low, int32_t j_high,
+                const nf_cellinfo *const *cellinfo, const float *const *pos_slab,
+                const uint64_t *nslab, float eps, float b2, int32_t softening, void *arena,
+                uint64_t arena_bytes, nf_sic *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NF_DIRECT_H */

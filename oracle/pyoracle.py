@@ -1,0 +1,206 @@
+"""ctypes access to the oracle libraries (liboracle.so, libcpudirects.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg -- never by the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+KIND = {"plummer": 0, "single_spline": 1, "cubic_spline": 2, "cubic_plummer": 3}
+
+_libs = {}
+
+
+def _load(name):
+    if name not in _libs:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        _libs[name] = C.CDLL(path)
+    return _libs[name]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+class _BoxArgs:
+    """Per-slab pointer tables of a box, kept alive with the arrays."""
+
+    def __init__(self, box):
+        s = box.slabs
+        self.keep = s
+        self.pos = _ptrs([x.pos for x in s])
+        self.nslab = np.array([x.n for x in s], np.int64)
+        self.start = _ptrs([x.start for x in s])
+        self.count = _ptrs([x.count for x in s])
+
+
+def kind_of(softening):
+    return KIND[softening] if isinstance(softening, str) else int(softening)
+
+
+# ---- pair kernels -----------------------------------------------------------
+
+def pair_f64(kind, d, eps, b2):
+    lib = _load("liboracle.so")
+    out = np.zeros(4)
+    lib.or_pair_f64(C.c_int(kind_of(kind)), C.c_double(d[0]), C.c_double(d[1]), C.c_double(d[2]),
+                    C.c_double(eps), C.c_double(b2), _p(out))
+    return out
+
+
+def pair_gpu_f32(kind, src, sink, eps, b2):
+    lib = _load("liboracle.so")
+    out = np.zeros(4, np.float32)
+    s = np.asarray(src, np.float32)
+    k = np.asarray(sink, np.float32)
+    lib.or_pair_gpu_f32(C.c_int(kind_of(kind)), _p(s), _p(k), C.c_float(eps), C.c_float(b2), _p(out))
+    return out
+
+
+def pair_cpu_f32(kind, src, sink, eps):
+    lib = _load("libcpudirects.so")
+    out = np.zeros(3, np.float32)
+    s = np.asarray(src, np.float32)
+    k = np.asarray(sink, np.float32)
+    lib.or_pair_cpu_f32(C.c_int(kind_of(kind)), _p(s), _p(k), C.c_float(eps), _p(out))
+    return out
+
+
+# ---- box-level oracles --------------------------------------------------------
+
+def direct_f64(box, R, kind, eps, b2, sink_slab, sink_index, sink_cell, with_soft=False):
+    """FP64 brute-force sum for the listed sinks -> (n, 4) float64 (and, with
+    ``with_soft``, the norm of the softened-pair part of each sum)."""
+    lib = _load("liboracle.so")
+    a = _BoxArgs(box)
+    ss = np.ascontiguousarray(sink_slab, np.int32)
+    si = np.ascontiguousarray(sink_index, np.int64)
+    sc = np.ascontiguousarray(sink_cell, np.int32)
+    out = np.zeros((ss.size, 4))
+    soft = np.zeros(ss.size) if with_soft else None
+    lib.or_direct_f64(C.c_int(box.cpd), a.pos, _p(a.nslab), a.start, a.count, C.c_int(R),
+                      C.c_int(kind_of(kind)), C.c_double(float(eps)), C.c_double(float(b2)),
+                      C.c_int64(ss.size), _p(ss), _p(si), _p(sc), _p(out),
+                      _p(soft) if with_soft else None)
+    return (out, soft) if with_soft else out
+
+
+def slab_sinks(box, x):
+    """(slab, index, cell) arrays for every particle of slab x."""
+    s = box.slab(x)
+    cell = np.repeat(np.arange(box.cpd * box.cpd, dtype=np.int32), s.count)
+    return np.full(s.n, x % box.cpd, np.int32), np.arange(s.n, dtype=np.int64), cell
+
+
+def sample_sinks(box, frac, seed):
+    """A seeded random subsample of all sinks of the box."""
+    rng = np.random.default_rng(seed)
+    sl, ix, ce = [], [], []
+    for x in range(box.cpd):
+        s = box.slab(x)
+        k = max(1, int(round(frac * s.n))) if s.n else 0
+        if k == 0:
+            continue
+        pick = np.sort(rng.choice(s.n, size=k, replace=False))
+        cell = np.repeat(np.arange(box.cpd * box.cpd, dtype=np.int32), s.count)
+        sl.append(np.full(k, x, np.int32))
+        ix.append(pick.astype(np.int64))
+        ce.append(cell[pick])
+    return np.concatenate(sl), np.concatenate(ix), np.concatenate(ce)
+
+
+def calc_di(box, R, x):
+    lib = _load("liboracle.so")
+    lib.or_calc_di.restype = C.c_uint64
+    a = _BoxArgs(box)
+    return int(lib.or_calc_di(C.c_int(box.cpd), a.count, C.c_int(R), C.c_int(x)))
+
+
+def execute_slab_cpu(box, R, kind, eps, x, y0=0, y1=None):
+    """ExecuteSlabCPU restatement -> ((n, 4) float32 acc, interactions)."""
+    lib = _load("libcpudirects.so")
+    lib.or_execute_slab_cpu.restype = C.c_uint64
+    a = _BoxArgs(box)
+    if y1 is None:
+        y1 = box.cpd
+    acc = np.zeros((box.slab(x).n, 4), np.float32)
+    di = lib.or_execute_slab_cpu(C.c_int(box.cpd), a.pos, _p(a.nslab), a.start, a.count, C.c_int(R),
+                                 C.c_int(kind_of(kind)), C.c_float(eps), C.c_int(x % box.cpd), C.c_int(y0),
+                                 C.c_int(y1), _p(acc))
+    return acc, int(di)
+
+
+def num_threads():
+    return int(_load("libcpudirects.so").or_num_threads())
+
+
+# ---- pencil-level restatements ---------------------------------------------
+
+def gather_sinks(plan, box, slab, R):
+    lib = _load("liboracle.so")
+    s = box.slab(slab)
+    W = 2 * R + 1
+    out = np.zeros((plan["NSinkBlocks"] * 64, 4), np.float32)
+    st = np.ascontiguousarray(plan["SinkPlanStart"], np.uint32)
+    nn = np.ascontiguousarray(plan["SinkPlanN"], np.int32)
+    wc = np.ascontiguousarray(plan["WrapCell"], np.int32)
+    ss = np.ascontiguousarray(plan["SinkSetStart"], np.int32)
+    lib.or_gather_sinks(C.c_int(plan["NSinkSets"]), C.c_int(W), C.c_int(R), _p(st), _p(nn), _p(wc), _p(ss),
+                        _p(s.pos), C.c_int64(s.n), C.c_float(np.float32(1.0 / box.cpd)), _p(out))
+    return out
+
+
+def gather_sources(plan, box, slab, R):
+    lib = _load("liboracle.so")
+    W = 2 * R + 1
+    slabs = [box.slab(slab - R + c) for c in range(W)]
+    out = np.zeros((plan["NSourceBlocks"] * 64, 4), np.float32)
+    st = np.ascontiguousarray(plan["SourcePlanStart"], np.uint32)
+    nn = np.ascontiguousarray(plan["SourcePlanN"], np.int32)
+    ss = np.ascontiguousarray(plan["SourceSetStart"], np.int32)
+    nsl = np.array([x.n for x in slabs], np.int64)
+    lib.or_gather_sources(C.c_int(plan["NSourceSets"]), C.c_int(W), C.c_int(R), _p(st), _p(nn), _p(ss),
+                          _ptrs([x.pos for x in slabs]), _p(nsl), C.c_float(np.float32(1.0 / box.cpd)), _p(out))
+    return out
+
+
+def coadd_f32(plan, pinacc, nslab, R):
+    lib = _load("liboracle.so")
+    W = 2 * R + 1
+    acc = np.zeros((nslab, 4), np.float32)
+    st = np.ascontiguousarray(plan["SinkPlanStart"], np.uint32)
+    nn = np.ascontiguousarray(plan["SinkPlanN"], np.int32)
+    wc = np.ascontiguousarray(plan["WrapCell"], np.int32)
+    ss = np.ascontiguousarray(plan["SinkSetStart"], np.int32)
+    sc = np.ascontiguousarray(plan["SinkSetCount"], np.int32)
+    pa = np.ascontiguousarray(pinacc, np.float32)
+    lib.or_coadd_f32(C.c_int(plan["NSinkSets"]), C.c_int(plan["Nk"]), C.c_int(W), _p(st), _p(nn), _p(wc), _p(ss),
+                     _p(sc), _p(pa), _p(acc))
+    return acc
+
+
+def pencil_execute(plan, sinks4, sources4, R, kind, eps, b2):
+    lib = _load("liboracle.so")
+    W = 2 * R + 1
+    out = np.zeros((plan["NSinkBlocks"] * 64, 4), np.float32)
+    par = np.ascontiguousarray(plan["SinkBlockParentPencil"], np.int32)
+    idm = np.ascontiguousarray(plan["SinkSetIdMax"], np.uint32)
+    ss = np.ascontiguousarray(plan["SourceSetStart"], np.int32)
+    sc = np.ascontiguousarray(plan["SourceSetCount"], np.int32)
+    il = np.ascontiguousarray(plan["SinkSourceInteractionList"], np.int32)
+    yo = np.ascontiguousarray(plan["SinkSourceYOffset"], np.float32)
+    k4 = np.ascontiguousarray(sinks4, np.float32)
+    s4 = np.ascontiguousarray(sources4, np.float32)
+    lib.or_pencil_execute(C.c_int(kind_of(kind)), C.c_int(W), C.c_int(plan["NSinkBlocks"]), C.c_int(64), _p(par),
+                          _p(idm), _p(ss), _p(sc), _p(il), _p(yo), _p(k4), _p(s4), C.c_float(eps), C.c_float(b2),
+                          _p(out))
+    return out

@@ -1,0 +1,164 @@
+"""GPU parity of the HIP near-field path against the oracle (run with -m gpu).
+
+Every test calls the product through the C ABI (libnfdirect.so via
+nearfield.py) and checks it against oracle/ on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+import nearfield as nfm
+import pyoracle as po
+import synth
+from nfhelpers import R, not_soft_dominated, oracle_slab, params, rel_err, small_uniform, summarize
+
+pytestmark = pytest.mark.gpu
+
+
+def run_box(box, path, softening="single_spline", slabs=None, nsplit=1, sinks_per_lane=0, bpd=2):
+    """Run the listed slabs of a host-resident box; returns {slab: acc (n,4)}."""
+    eps, b2, _ = params(box, softening)
+    ell = box.mean_spacing()
+    out = {}
+    with nfm.NearFieldDriver(box.cpd, ell / 40.0, b2, nfradius=R, softening=softening, path=path,
+                             buffers_per_device=bpd, max_buffer_bytes=2 << 30,
+                             sinks_per_lane=sinks_per_lane) as drv:
+        assert drv.eps == eps
+        slabs = range(box.cpd) if slabs is None else slabs
+        for x in slabs:
+            acc = np.full((box.slab(x).n, 4), np.nan, np.float32)
+            drv.execute_slab(box, x, acc, nsplit=nsplit)
+            out[x] = acc
+        for x in slabs:
+            drv.finalize(x)
+    return out
+
+
+@pytest.fixture(scope="module")
+def box9():
+    return small_uniform(cpd=9, ppc=20, seed=11)
+
+
+@pytest.fixture(scope="module")
+def ref9(box9):
+    eps, b2, _ = params(box9)
+    return {x: oracle_slab(box9, x, "single_spline", eps, b2, with_soft=True) for x in (0, 4, 8)}
+
+
+def test_plain_matches_pencil_restatement(box9):
+    """Plain path == the pencil-structured FP32 restatement (same pencil
+    decomposition, tile sums and co-add order), up to rsqrt rounding."""
+    eps, b2, _ = params(box9)
+    acc = run_box(box9, "plain", slabs=[0, 4, 8])
+    for x in (0, 4, 8):
+        sic = nfm.SetInteractionCollection(box9, x, 0, box9.cpd, R, eps, b2, "single_spline")
+        plan = sic.arrays()
+        k4 = po.gather_sinks(plan, box9, x, R)
+        s4 = po.gather_sources(plan, box9, x, R)
+        part = po.pencil_execute(plan, k4, s4, R, "single_spline", eps, b2)
+        ref = po.coadd_f32(plan, part, box9.slab(x).n, R)
+        assert np.isfinite(acc[x]).all()
+        st = summarize(rel_err(acc[x], ref))
+        assert st["rms"] < 1e-6 and st["max"] < 2e-5, st
+        np.testing.assert_allclose(acc[x][:, 3], ref[:, 3], rtol=1e-5, atol=1e-6 * float(b2))
+
+
+def test_plain_and_batched_vs_fp64(box9, ref9):
+    eps, b2, e = params(box9)
+    res = {p: run_box(box9, p, slabs=[0, 4, 8]) for p in ("plain", "batched")}
+    for x in (0, 4, 8):
+        ref, soft = ref9[x]
+        keep = not_soft_dominated(ref, soft)
+        for p in ("plain", "batched"):
+            acc = res[p][x]
+            assert np.isfinite(acc).all(), p
+            st = summarize(rel_err(acc, ref), keep)
+            lim_rms, lim_max = (3e-6, 1e-4) if p == "plain" else (1e-6, 1e-5)
+            assert st["rms"] < lim_rms and st["max"] < lim_max, (p, x, st)
+            # FOF density w (self term b2 included) against the FP64 sum
+            np.testing.assert_allclose(acc[:, 3], ref[:, 3], rtol=2e-6, atol=1e-6 * float(b2))
+
+
+@pytest.mark.parametrize("S", [1, 2, 4])
+def test_batched_register_tiles_agree(box9, ref9, S):
+    """Every register-tile shape gives the same physics (within FP32)."""
+    acc = run_box(box9, "batched", slabs=[4], sinks_per_lane=S)[4]
+    ref, soft = ref9[4]
+    st = summarize(rel_err(acc, ref), not_soft_dominated(ref, soft))
+    assert st["rms"] < 1e-6 and st["max"] < 1e-5, (S, st)
+
+
+def test_split_sics_equal_whole_slab(box9):
+    """A slab planned as 3 SICs over y gives bitwise the same accelerations
+    as one SIC (each SIC covers whole skewers; co-add is per skewer)."""
+    for path in ("plain", "batched"):
+        a1 = run_box(box9, path, slabs=[2], nsplit=1)[2]
+        a3 = run_box(box9, path, slabs=[2], nsplit=3)[2]
+        assert np.array_equal(a1, a3), path
+
+
+def test_deterministic(box9):
+    a = run_box(box9, "batched", slabs=[1])[1]
+    b = run_box(box9, "batched", slabs=[1])[1]
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("softening", ["plummer", "cubic_spline", "cubic_plummer"])
+def test_other_softenings_vs_fp64(softening):
+    box = small_uniform(cpd=7, ppc=15, seed=5)
+    eps, b2, _ = params(box, softening)
+    for path in ("plain", "batched"):
+        acc = run_box(box, path, softening=softening, slabs=[3])[3]
+        ref, soft = oracle_slab(box, 3, softening, eps, b2, with_soft=True)
+        st = summarize(rel_err(acc, ref), not_soft_dominated(ref, soft))
+        assert st["rms"] < 3e-6 and st["max"] < 1e-4, (softening, path, st)
+
+
+def test_empty_and_heavy_cells():
+    """Ragged occupancy: empty cells, an empty pencil region and one heavy
+    cell (1,500 particles) next to it."""
+    cpd = 7
+    rng = np.random.default_rng(3)
+    pts = [rng.random((cpd ** 3 * 4, 3)) - 0.5]
+    pts[0] = pts[0][pts[0][:, 0] > -0.2]                       # slabs 0-1 empty
+    pts.append(rng.normal(0.0, 0.01, (1500, 3)))                # heavy cell at the centre
+    box = synth.box_from_global(np.concatenate(pts), cpd)
+    assert box.slab(0).n == 0 and box.counts().max() >= 1000
+    eps, b2, _ = params(box)
+    for path in ("plain", "batched"):
+        res = run_box(box, path, slabs=[0, 1, 3])
+        assert res[0].shape == (0, 4)
+        for x in (1, 3):
+            ref, soft = oracle_slab(box, x, "single_spline", eps, b2, with_soft=True)
+            st = summarize(rel_err(res[x], ref), not_soft_dominated(ref, soft))
+            assert st["rms"] < 3e-6, (path, x, st)
+
+
+def test_minimum_cpd_wraps():
+    """cpd = 2R+1: every pencil wraps, every source slab is distinct."""
+    box = synth.uniform_box(5 ** 3 * 12, 5, seed=9)
+    eps, b2, _ = params(box)
+    for path in ("plain", "batched"):
+        acc = run_box(box, path, slabs=[0])[0]
+        ref, soft = oracle_slab(box, 0, "single_spline", eps, b2, with_soft=True)
+        st = summarize(rel_err(acc, ref), not_soft_dominated(ref, soft))
+        assert st["rms"] < 3e-6, (path, st)
+
+
+def test_device_resident_matches_host():
+    """NF_MEM_DEVICE (slabs and acceleration slab in HBM, torch-allocated)
+    gives bitwise the host-mode result."""
+    import torch
+    box = small_uniform(cpd=7, ppc=20, seed=21)
+    eps, b2, _ = params(box)
+    host = run_box(box, "batched", slabs=[3])[3]
+    dev = [torch.from_numpy(box.slab(x).pos).cuda() for x in range(box.cpd)]
+    acc = torch.zeros((box.slab(3).n, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                             max_buffer_bytes=1 << 30) as drv:
+        ptrs = [dev[(3 - R + c) % box.cpd].data_ptr() for c in range(2 * R + 1)]
+        sic = drv.plan(box, 3, pos_ptrs=ptrs, acc_ptr=acc.data_ptr(), mem=nfm.NF_MEM_DEVICE)
+        t = drv.submit(sic)
+        t.wait()
+        t.release()
+    assert np.array_equal(acc.cpu().numpy(), host)

@@ -1,0 +1,244 @@
+"""Pins the oracle: closed forms, symmetry invariants, brute-force counts.
+
+The oracle (oracle/) is what every GPU result is checked against, so these
+tests check it against mathematics that does not depend on its own code:
+the force laws' closed forms, Newton's third law, lattice symmetry, an
+independent numpy interaction count, and a planted mistake that must fail.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import synth
+from nfhelpers import R, params, rel_err, small_uniform, summarize
+
+
+# ---------------------------------------------------------------- pair forms
+
+@pytest.mark.parametrize("u", [0.05, 0.3, 0.7, 0.999, 1.0, 1.001, 1.5, 3.0, 10.0])
+def test_single_spline_closed_form(u):
+    """F(u) = 10 - 15u + 6u^2 (u <= 1), u^-3 (u > 1), a = -F d (d = src - sink)
+    in units where the e^-3 prefactor is applied later (kick.cpp:88-99)."""
+    e = 1e-3
+    eps = 1.0 / (e * e)
+    d = np.array([u * e, 0.0, 0.0])
+    a = po.pair_f64("single_spline", d, eps, 0.0)
+    F = (10 - 15 * u + 6 * u * u) if u <= 1 else u ** -3
+    assert a[0] == pytest.approx(-F * d[0], rel=1e-13)
+    assert a[1] == 0 and a[2] == 0
+    if u > 1:  # Newtonian: |a| e^-3 = 1/r^2
+        assert abs(a[0]) / e ** 3 == pytest.approx(1.0 / (u * e) ** 2, rel=1e-12)
+
+
+def test_single_spline_is_c1_at_u1():
+    """Both branches give F = 1 and F' = -3 at u = 1 (the spline's matching)."""
+    poly = lambda u: 10 - 15 * u + 6 * u * u
+    assert poly(1.0) == 1.0
+    h = 1e-6
+    assert (poly(1 + h) - poly(1 - h)) / (2 * h) == pytest.approx(-3.0, rel=1e-6)
+    e = 1e-3
+    lo = po.pair_f64("single_spline", [(1 - 1e-9) * e, 0, 0], 1 / e ** 2, 0)[0]
+    hi = po.pair_f64("single_spline", [(1 + 1e-9) * e, 0, 0], 1 / e ** 2, 0)[0]
+    assert lo == pytest.approx(hi, rel=1e-7)
+
+
+@pytest.mark.parametrize("r", [1e-4, 1e-3, 2.5e-3, 1e-2])
+def test_plummer_closed_form(r):
+    e = 1e-3
+    a = po.pair_f64("plummer", [0, r, 0], e * e, 0)
+    assert -a[1] == pytest.approx(r / (r * r + e * e) ** 1.5, rel=1e-13)
+
+
+@pytest.mark.parametrize("u", [0.2, 0.99, 1.0, 1.5, 1.999, 2.0, 3.0])
+def test_cubic_spline_branches(u):
+    """Hernquist & Katz 1989 kernel: Newtonian for u >= 2, continuous at 1 and 2."""
+    e = 1e-3
+    eps = 1.0 / e
+    def g(uu):
+        return -po.pair_f64("cubic_spline", [0, 0, uu * e], eps, 0)[2] / (uu * e)
+    if u >= 2:
+        assert g(u) == pytest.approx((u * e) ** -3, rel=1e-12)
+    for knot in (1.0, 2.0):
+        assert g(knot * (1 - 1e-9)) == pytest.approx(g(knot * (1 + 1e-9)), rel=1e-6)
+
+
+def test_cubic_plummer_closed_form():
+    e = 2e-3
+    for r in (1e-4, 1e-3, 1e-2):
+        a = po.pair_f64("cubic_plummer", [r, 0, 0], e ** 3, 0)
+        assert -a[0] == pytest.approx(r / ((r * r + 1e-16) ** 1.5 + e ** 3), rel=1e-12)
+
+
+def test_density_term_and_self_pair():
+    b2 = 4e-6
+    e = 1e-3
+    for kind in ("single_spline", "plummer"):
+        eps = 1 / e ** 2 if kind == "single_spline" else e * e
+        a = po.pair_f64(kind, [0, 0, 0], eps, b2)
+        assert np.array_equal(a, [0, 0, 0, b2])           # self pair: no force, w += b2
+        a = po.pair_f64(kind, [1e-3, 0, 0], eps, b2)
+        assert a[3] == pytest.approx(b2 - 1e-6, rel=1e-14)  # inside b
+        a = po.pair_f64(kind, [3e-3, 0, 0], eps, b2)
+        assert a[3] == 0.0                                    # outside b
+
+
+def test_pair_antisymmetry():
+    rng = np.random.default_rng(0)
+    e = 1e-3
+    for kind, eps in (("single_spline", 1 / e ** 2), ("plummer", e * e), ("cubic_spline", 1 / e),
+                      ("cubic_plummer", e ** 3)):
+        for _ in range(20):
+            d = rng.normal(0, 2e-3, 3)
+            np.testing.assert_allclose(po.pair_f64(kind, d, eps, 0)[:3], -po.pair_f64(kind, -d, eps, 0)[:3],
+                                       rtol=1e-15, atol=0)
+
+
+def test_fp32_forms_track_fp64():
+    """The FP32 restatements (GPU form and -Ofast CPU form) stay within a few
+    ULP-scale relative errors of the FP64 form on random pairs."""
+    rng = np.random.default_rng(1)
+    e = 8.4e-4
+    eps = np.float32(1 / e ** 2)
+    worst_gpu = worst_cpu = 0.0
+    for _ in range(2000):
+        sink = rng.uniform(-0.03, 0.03, 3).astype(np.float32)
+        src = (sink + rng.normal(0, 0.01, 3)).astype(np.float32)
+        d = src.astype(np.float64) - sink.astype(np.float64)
+        ref = po.pair_f64("single_spline", d, float(eps), 0)[:3]
+        g = po.pair_gpu_f32("single_spline", src, sink, eps, 0)[:3]
+        c = po.pair_cpu_f32("single_spline", src, sink, eps)
+        n = np.linalg.norm(ref)
+        worst_gpu = max(worst_gpu, np.linalg.norm(g - ref) / n)
+        worst_cpu = max(worst_cpu, np.linalg.norm(c - ref) / n)
+    assert worst_gpu < 2e-6 and worst_cpu < 2e-6, (worst_gpu, worst_cpu)
+
+
+# ----------------------------------------------------------------- box level
+
+def numpy_calc_di(box, x):
+    """Independent interaction count: n_sink * (box-filtered source counts)."""
+    cpd = box.cpd
+    cnt = box.counts().reshape(cpd, cpd, cpd).astype(np.int64)
+    nb = np.zeros_like(cnt)
+    for a in range(-R, R + 1):
+        for b in range(-R, R + 1):
+            for c in range(-R, R + 1):
+                nb += np.roll(cnt, (-a, -b, -c), axis=(0, 1, 2))
+    return int((cnt[x] * nb[x]).sum())
+
+
+def test_calc_di_matches_numpy():
+    box = small_uniform(cpd=7, ppc=9, seed=4)
+    for x in (0, 3, 6):
+        assert po.calc_di(box, R, x) == numpy_calc_di(box, x)
+
+
+def two_particle_box(cpd, p, q):
+    return synth.box_from_global(np.array([p, q], np.float64), cpd)
+
+
+@pytest.mark.parametrize("p,q", [
+    ([0.01, 0.02, -0.03], [0.06, -0.01, 0.04]),       # neighbouring cells
+    ([-0.49, 0.1, 0.1], [0.47, 0.12, 0.09]),          # across the periodic wrap in x
+    ([0.2, -0.495, 0.3], [0.2, 0.49, 0.28]),          # across the wrap in y
+    ([0.0, 0.0, 0.496], [0.01, 0.0, -0.47]),          # across the wrap in z
+])
+def test_fp64_two_particles(p, q):
+    """Two particles: each sees only the other (minimum image within the
+    neighbourhood), a Newtonian pair force along the true separation."""
+    cpd = 9
+    box = two_particle_box(cpd, p, q)
+    e = 1e-4
+    eps = 1 / e ** 2
+    out = {}
+    for x in range(cpd):
+        s = box.slab(x)
+        if s.n:
+            sl, ix, ce = po.slab_sinks(box, x)
+            res = po.direct_f64(box, R, "single_spline", eps, 0.0, sl, ix, ce)
+            glob = (np.stack([np.full(s.n, x), ce // cpd, ce % cpd], 1) - (cpd - 1) // 2) / cpd + s.xyz
+            for g, a in zip(glob, res):
+                out[tuple(np.round(g, 6))] = a
+    pa, qa = out[tuple(np.round(p, 6))], out[tuple(np.round(q, 6))]
+    d = np.array(q) - np.array(p)
+    d -= np.round(d)                                   # minimum image
+    r = np.linalg.norm(d)
+    expect = -d / r ** 3 * e ** 3                       # a = -F(u) d with F = u^-3 (units: e^3/r^3)
+    np.testing.assert_allclose(pa[:3], expect, rtol=2e-6)
+    np.testing.assert_allclose(qa[:3], -expect, rtol=2e-6)
+
+
+def test_fp64_lattice_has_zero_force():
+    """A particle at every cell centre: each sink's neighbourhood is a
+    symmetric cube, so the net force vanishes (CheckForces' lattice idea)."""
+    cpd = 7
+    g = (np.stack(np.meshgrid(*[np.arange(cpd)] * 3, indexing="ij"), -1).reshape(-1, 3) - 3) / cpd
+    box = synth.box_from_global(g + 1e-12, cpd)
+    e = 1e-3
+    sl, ix, ce = po.slab_sinks(box, 2)
+    a = po.direct_f64(box, R, "single_spline", 1 / e ** 2, 0.0, sl, ix, ce)
+    typical = e ** 3 / (1.0 / cpd) ** 2
+    assert np.abs(a[:, :3]).max() < 1e-9 * typical
+
+
+def test_fp64_momentum_conservation():
+    """Neighbourhoods are symmetric, so pair forces cancel over the box."""
+    box = small_uniform(cpd=7, ppc=6, seed=8)
+    eps, b2, _ = params(box)
+    tot = np.zeros(3)
+    mag = 0.0
+    for x in range(box.cpd):
+        sl, ix, ce = po.slab_sinks(box, x)
+        a = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce)
+        tot += a[:, :3].sum(0)
+        mag += np.abs(a[:, :3]).sum()
+    assert np.linalg.norm(tot) < 1e-12 * mag
+
+
+def test_planted_offset_mistake_is_caught():
+    """Flip the sign of one cell offset in a copy of the oracle's inputs
+    (equivalently, mirror a source slab) and the two-particle check fails."""
+    cpd = 9
+    p, q = [0.01, 0.02, -0.03], [0.06, -0.01, 0.04]
+    box = two_particle_box(cpd, p, q)
+    x = 4
+    s = box.slab(x)
+    mirrored = synth.Box(cpd, [synth.Slab(np.concatenate([-s.xyz[:, 0], s.xyz[:, 1], s.xyz[:, 2]]), s.count)
+                               if i == x else box.slabs[i] for i in range(cpd)])
+    e = 1e-4
+    d = np.array(q) - np.array(p)
+    expect = -d / np.linalg.norm(d) ** 3 * e ** 3
+    for x2 in range(cpd):
+        if mirrored.slab(x2).n and x2 != x:
+            sl, ix, ce = po.slab_sinks(mirrored, x2)
+            a = po.direct_f64(mirrored, R, "single_spline", 1 / e ** 2, 0, sl, ix, ce)[0]
+            assert not np.allclose(a[:3], -expect, rtol=1e-3, atol=0)
+
+
+def test_cpu_fallback_restatement_vs_fp64():
+    box = small_uniform(cpd=9, ppc=15, seed=2)
+    eps, b2, _ = params(box)
+    for x in (0, 5):
+        acc, di = po.execute_slab_cpu(box, R, "single_spline", eps, x)
+        assert di == po.calc_di(box, R, x)
+        sl, ix, ce = po.slab_sinks(box, x)
+        ref = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce)
+        st = summarize(rel_err(acc, ref))
+        assert st["rms"] < 3e-6, st
+        assert np.all(acc[:, 3] == 0)  # the CPU fallback computes no density
+
+
+def test_pencil_restatement_matches_cell_pair():
+    """Pencil-structured (CPUExecute) and cell-pair (ExecuteSlabCPU) give the
+    same physics: the pencil decomposition covers each neighbour cell once."""
+    import nearfield as nfm  # planner from the product, checked separately
+    box = small_uniform(cpd=7, ppc=12, seed=6)
+    eps, b2, _ = params(box)
+    x = 2
+    plan = nfm.SetInteractionCollection(box, x, 0, box.cpd, R, eps, b2, "single_spline").arrays()
+    k4 = po.gather_sinks(plan, box, x, R)
+    s4 = po.gather_sources(plan, box, x, R)
+    acc_p = po.coadd_f32(plan, po.pencil_execute(plan, k4, s4, R, "single_spline", eps, b2), box.slab(x).n, R)
+    acc_c, _ = po.execute_slab_cpu(box, R, "single_spline", eps, x)
+    st = summarize(rel_err(acc_p, acc_c))
+    assert st["rms"] < 3e-6 and st["max"] < 1e-4, st
