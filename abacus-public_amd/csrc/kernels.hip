@@ -28,6 +28,23 @@ namespace {
 constexpr int kWave = 64;
 constexpr float kFar = 1.0e30f;  // x of a padding source: its pair terms are exact zeros
 
+// Read-only data of a launch is addressed through the constant address space:
+// the backend then loads wave-uniform addresses with scalar loads (s_load)
+// and divergent ones with global loads, even when the pointer itself came
+// from memory (the SIC view table) rather than from a kernel argument.
+// (The qualifiers exist only in the device compilation pass of this file.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define NF_CONST __attribute__((address_space(4)))
+#define NF_GLOBAL __attribute__((address_space(1)))
+#else
+#define NF_CONST
+#define NF_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ const NF_CONST T *cptr(const void *p) {
+    return (const NF_CONST T *)(p);
+}
+
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 
@@ -293,21 +310,27 @@ __global__ __launch_bounds__(64) void k_coadd_f64(DevPlan pl, const double *__re
 // ---------------------------------------------------------------------------
 
 // Pair kernel in box units.  r2 carries pc.r2add; for the single spline the
-// result is g/e^3 (the e^3 is restored in the co-add).  FAST evaluates the
-// unsoftened branch only and is valid when no pair of the tile is softened.
-template <int SOFT, bool FAST>
+// result is g/e^3 (the e^3 is restored in the co-add).  MODE selects what a
+// source tile needs, decided from the tile's bounding box:
+//   kNoDensity  unsoftened branch only, no FOF density (no pair within b)
+//   kDensity    unsoftened branch only, with the FOF density term
+//   kExact      the full softened form with the density term
+// The unsoftened branch equals the full form for every pair with r > e.
+constexpr int kNoDensity = 0, kDensity = 1, kExact = 2;
+
+template <int SOFT, int MODE>
 __device__ __forceinline__ void pair_box(float sx, float sy, float sz, float kx, float ky, float kz,
                                          const PairConst &pc, float &ax, float &ay, float &az, float &aw) {
     const float dx = sx - kx, dy = sy - ky, dz = sz - kz;
     float r2 = fmaf(dx, dx, pc.r2add);
     r2 = fmaf(dy, dy, r2);
     r2 = fmaf(dz, dz, r2);
-    aw += clamp01(pc.b2x - r2);
+    if constexpr (MODE != kNoDensity) aw += clamp01(pc.b2x - r2);
     float g;
     if constexpr (SOFT == NF_SOFT_SINGLE_SPLINE) {
         const float f = rsq(r2);
         const float f3 = f * f * f;
-        if constexpr (FAST) g = f3;
+        if constexpr (MODE != kExact) g = f3;
         else g = fminf(f3, fmaf(fmaf(pc.c1, f, pc.c2), r2, pc.c3));
     } else if constexpr (SOFT == NF_SOFT_PLUMMER) {
         const float f = rsq(r2);
@@ -358,26 +381,61 @@ __device__ __forceinline__ float box_gap2(float alx, float aly, float alz, float
 }
 
 struct BatchCtx {
-    DevPlan pl;
     PairConst pc;
     double invcpd;
     float soft2;  // squared distance above which the unsoftened branch is exact (with margin)
+    float dens2;  // squared distance above which the FOF density term is zero (with margin)
 };
+
+// The mode a source tile needs, from the squared gap between its bounding box
+// and the sinks' bounding box (wave-uniform).
+template <int SOFT>
+__device__ __forceinline__ int tile_mode(const BatchCtx &cx, float gap2) {
+    if constexpr (SOFT == NF_SOFT_CUBIC_SPLINE || SOFT == NF_SOFT_CUBIC_PLUMMER) return kExact;
+    if constexpr (SOFT == NF_SOFT_SINGLE_SPLINE)
+        if (!cx.pc.fast_ok || gap2 <= cx.soft2) return kExact;
+    return gap2 <= cx.dens2 ? kDensity : kNoDensity;
+}
+
+// One source tile (cnt8 <= 64 sources, a multiple of 8) against the S sinks
+// of every lane; sources are wave-uniform loads (scalar cache).
+template <int SOFT, int MODE, int S>
+__device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, const float *kx, const float *ky,
+                                          const float *kz, const PairConst &pc, float *tx, float *ty, float *tz,
+                                          float *tw) {
+    for (int i = 0; i < cnt8; i += 8) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const float4 sv = sp[i + q];
+#pragma unroll
+            for (int u = 0; u < S; u++)
+                pair_box<SOFT, MODE>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, tx[u], ty[u], tz[u], tw[u]);
+        }
+    }
+}
 
 // Register tile: 64*S sinks of one pencil on the lanes; sources are wave
 // uniform and come through the scalar cache.
 template <int SOFT, int S>
-__device__ __forceinline__ void full_item(const BatchCtx &cx, const WorkItem &w, int lane,
-                                          const float4 *__restrict__ sinks, const float4 *__restrict__ src,
-                                          const float4 *__restrict__ box, double *__restrict__ out) {
-    const DevPlan &pl = cx.pl;
+__device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane) {
+    const NF_CONST float4 *sinks = cptr<float4>(v.sinks4);
+    const NF_CONST float4 *src = cptr<float4>(v.sources4);
+    const NF_CONST float4 *box = cptr<float4>(v.box4);
+    const NF_CONST int32_t *sink_set_start = cptr<int32_t>(v.sink_set_start);
+    const NF_CONST int32_t *src_set_start = cptr<int32_t>(v.src_set_start);
+    const NF_CONST int32_t *src_set_count = cptr<int32_t>(v.src_set_count);
+    const NF_CONST int32_t *ilist = cptr<int32_t>(v.ilist);
+    NF_GLOBAL double *out = (NF_GLOBAL double *)v.partial;
     const int s = w.sinkset;
-    const int base = pl.sink_set_start[s] + w.first;
+    const int base = sink_set_start[s] + w.first;
+    const int nsink = w.nsink;  // 64*S, or fewer for a masked tail tile
     float kx[S], ky[S], kz[S];
     float blx = __builtin_huge_valf(), bly = blx, blz = blx, bhx = -blx, bhy = -blx, bhz = -blx;
 #pragma unroll
     for (int u = 0; u < S; u++) {
-        const float4 k = sinks[base + lane + kWave * u];
+        // Lanes past nsink repeat sink 0 (inside the box, results not stored).
+        const int ki = lane + kWave * u < nsink ? lane + kWave * u : 0;
+        const float4 k = sinks[base + ki];
         kx[u] = k.x;
         ky[u] = k.y;
         kz[u] = k.z;
@@ -399,52 +457,33 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const WorkItem &w,
 #pragma unroll
     for (int u = 0; u < S; u++) hx[u] = hy[u] = hz[u] = lx[u] = ly[u] = lz[u] = hw[u] = 0.f;
 
-    const int W = pl.W;
+    const int W = v.W;
     for (int y = 0; y < W; y++) {
-        const int si = pl.ilist[s * W + y];
-        const int st = pl.src_set_start[si];
-        const int n = pl.src_set_count[si];
+        const int si = ilist[s * W + y];
+        const int st = src_set_start[si];
+        const int n = src_set_count[si];
         // Source y + (y-R)/cpd == sink y - (y-R)/cpd on the other side; the
         // cell-centre offset is formed in FP64.
-        const double yo = (double)(y - pl.R) * cx.invcpd;
+        const double yo = (double)(y - v.R) * cx.invcpd;
         float kyy[S];
 #pragma unroll
         for (int u = 0; u < S; u++) kyy[u] = (float)((double)ky[u] - yo);
         const float sly = (float)((double)bly - yo), shy = (float)((double)bhy - yo);
         const int nblk = (n + kWave - 1) / kWave;
         for (int b = 0; b < nblk; b++) {
-            const float4 *sp = src + st + b * kWave;
+            const NF_CONST float4 *sp = src + st + b * kWave;
             const int cnt8 = (min(kWave, n - b * kWave) + 7) & ~7;
             const int64_t gb = (int64_t)st / kWave + b;
-            const bool fast = (SOFT == NF_SOFT_PLUMMER) ||
-                              (SOFT == NF_SOFT_SINGLE_SPLINE && cx.pc.fast_ok &&
-                               box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]) > cx.soft2);
+            const int mode = tile_mode<SOFT>(cx, box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]));
             float tx[S], ty[S], tz[S], tw[S];
 #pragma unroll
             for (int u = 0; u < S; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
-            if (fast) {
-                for (int i = 0; i < cnt8; i += 8) {
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const float4 sv = sp[i + q];
-#pragma unroll
-                        for (int u = 0; u < S; u++)
-                            pair_box<SOFT, true>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
-                                                 tw[u]);
-                    }
-                }
-            } else {
-                for (int i = 0; i < cnt8; i += 8) {
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const float4 sv = sp[i + q];
-#pragma unroll
-                        for (int u = 0; u < S; u++)
-                            pair_box<SOFT, false>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
-                                                  tw[u]);
-                    }
-                }
-            }
+            if (mode == kNoDensity)
+                full_tile<SOFT, kNoDensity, S>(sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
+            else if (mode == kDensity)
+                full_tile<SOFT, kDensity, S>(sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
+            else
+                full_tile<SOFT, kExact, S>(sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
 #pragma unroll
             for (int u = 0; u < S; u++) {
                 merge(hx[u], lx[u], tx[u]);
@@ -456,7 +495,8 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const WorkItem &w,
     }
 #pragma unroll
     for (int u = 0; u < S; u++) {
-        double *o = out + 4 * (int64_t)(base + lane + kWave * u);
+        if (lane + kWave * u >= nsink) continue;
+        NF_GLOBAL double *o = out + 4 * (int64_t)(base + lane + kWave * u);
         o[0] = (double)hx[u] + (double)lx[u];
         o[1] = (double)hy[u] + (double)ly[u];
         o[2] = (double)hz[u] + (double)lz[u];
@@ -482,15 +522,20 @@ __device__ __forceinline__ float wave_sum(float v) {
 // Remainder of a pencil (fewer than 64*S sinks): sources on the lanes, SU
 // sinks wave-uniform per pass, per-lane partial sums reduced across the wave.
 template <int SOFT, int SU>
-__device__ __forceinline__ void rem_item(const BatchCtx &cx, const WorkItem &w, int lane,
-                                         const float4 *__restrict__ sinks, const float4 *__restrict__ src,
-                                         const float4 *__restrict__ box, double *__restrict__ out) {
+__device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane) {
     constexpr int TC = 16;  // chunks of 64 sources per TwoSum tile
-    const DevPlan &pl = cx.pl;
+    const NF_CONST float4 *sinks = cptr<float4>(v.sinks4);
+    const NF_CONST float4 *src = cptr<float4>(v.sources4);
+    const NF_CONST float4 *box = cptr<float4>(v.box4);
+    const NF_CONST int32_t *sink_set_start = cptr<int32_t>(v.sink_set_start);
+    const NF_CONST int32_t *src_set_start = cptr<int32_t>(v.src_set_start);
+    const NF_CONST int32_t *src_set_count = cptr<int32_t>(v.src_set_count);
+    const NF_CONST int32_t *ilist = cptr<int32_t>(v.ilist);
+    NF_GLOBAL double *out = (NF_GLOBAL double *)v.partial;
     const int s = w.sinkset;
-    const int base = pl.sink_set_start[s] + w.first;
+    const int base = sink_set_start[s] + w.first;
     const int r = w.nsink;
-    const int W = pl.W;
+    const int W = v.W;
     for (int g = 0; g < r; g += SU) {
         float kx[SU], ky[SU], kz[SU];
         float blx = __builtin_huge_valf(), bly = blx, blz = blx, bhx = -blx, bhy = -blx, bhz = -blx;
@@ -512,10 +557,10 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const WorkItem &w, 
 #pragma unroll
         for (int u = 0; u < SU; u++) hx[u] = hy[u] = hz[u] = lx[u] = ly[u] = lz[u] = hw[u] = 0.f;
         for (int y = 0; y < W; y++) {
-            const int si = pl.ilist[s * W + y];
-            const int st = pl.src_set_start[si];
-            const int n = pl.src_set_count[si];
-            const double yo = (double)(y - pl.R) * cx.invcpd;
+            const int si = ilist[s * W + y];
+            const int st = src_set_start[si];
+            const int n = src_set_count[si];
+            const double yo = (double)(y - v.R) * cx.invcpd;
             float kyy[SU];
 #pragma unroll
             for (int u = 0; u < SU; u++) kyy[u] = (float)((double)ky[u] - yo);
@@ -529,19 +574,23 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const WorkItem &w, 
                 for (int b = b0; b < b1; b++) {
                     const float4 sv = src[st + b * kWave + lane];
                     const int64_t gb = (int64_t)st / kWave + b;
-                    const bool fast = (SOFT == NF_SOFT_PLUMMER) ||
-                                      (SOFT == NF_SOFT_SINGLE_SPLINE && cx.pc.fast_ok &&
-                                       box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]) > cx.soft2);
-                    if (fast) {
+                    const int mode =
+                        tile_mode<SOFT>(cx, box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]));
+                    if (mode == kNoDensity) {
 #pragma unroll
                         for (int u = 0; u < SU; u++)
-                            pair_box<SOFT, true>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
-                                                 tw[u]);
+                            pair_box<SOFT, kNoDensity>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u],
+                                                       tz[u], tw[u]);
+                    } else if (mode == kDensity) {
+#pragma unroll
+                        for (int u = 0; u < SU; u++)
+                            pair_box<SOFT, kDensity>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u],
+                                                     tz[u], tw[u]);
                     } else {
 #pragma unroll
                         for (int u = 0; u < SU; u++)
-                            pair_box<SOFT, false>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
-                                                  tw[u]);
+                            pair_box<SOFT, kExact>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
+                                                   tw[u]);
                     }
                 }
 #pragma unroll
@@ -560,7 +609,7 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const WorkItem &w, 
             wave_two_sum(hz[u], lz[u]);
             const float ws = wave_sum(hw[u]);
             if (lane == 0 && g + u < r) {
-                double *o = out + 4 * (int64_t)(base + g + u);
+                NF_GLOBAL double *o = out + 4 * (int64_t)(base + g + u);
                 o[0] = (double)hx[u] + (double)lx[u];
                 o[1] = (double)hy[u] + (double)ly[u];
                 o[2] = (double)hz[u] + (double)lz[u];
@@ -572,21 +621,24 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const WorkItem &w, 
 
 constexpr int kRemSinks = 4;
 
-// Persistent kernel: every wave pops items until the queue is empty.
+// Persistent kernel: every wave pops items (of any SIC of the group) until
+// the queue is empty.
 template <int SOFT, int S>
-__global__ __launch_bounds__(256) void k_batched(BatchCtx cx, const float4 *__restrict__ sinks,
-                                                 const float4 *__restrict__ src, const float4 *__restrict__ box,
+__global__ __launch_bounds__(256) void k_batched(BatchCtx cx, const SicView *__restrict__ views,
                                                  const WorkItem *__restrict__ items, int32_t nitems,
-                                                 int32_t *__restrict__ counter, double *__restrict__ out) {
+                                                 int32_t *__restrict__ counter) {
     const int lane = threadIdx.x & (kWave - 1);
     for (;;) {
         int it = 0;
         if (lane == 0) it = atomicAdd(counter, 1);
         it = __builtin_amdgcn_readfirstlane(it);
         if (it >= nitems) return;
-        const WorkItem w = items[it];
-        if (w.kind == 0) full_item<SOFT, S>(cx, w, lane, sinks, src, box, out);
-        else rem_item<SOFT, kRemSinks>(cx, w, lane, sinks, src, box, out);
+        const WorkItem w = cptr<WorkItem>(items)[it];
+        const int kind = w.kind_sic & 3;
+        const SicView v = cptr<SicView>(views)[w.kind_sic >> 2];
+        if (kind == 0) full_item<SOFT, S>(cx, v, w, lane);
+        else if (kind == 2) full_item<SOFT, 1>(cx, v, w, lane);
+        else rem_item<SOFT, kRemSinks>(cx, v, w, lane);
     }
 }
 
@@ -604,11 +656,6 @@ int launch_plain_t(const DevPlan &pl, const PairConst &pc, const void *sinks4, c
 
 }  // namespace
 
-static float4 *box_of(void *sources4, int32_t nsrc_blocks) {
-    // Bounding boxes follow the padded source stream in the same allocation.
-    return static_cast<float4 *>(sources4) + (int64_t)nsrc_blocks * kWave;
-}
-
 int device_cu_count() {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -621,7 +668,7 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void
     hipStream_t st = static_cast<hipStream_t>(stream);
     const float cellsize = (float)(1.0 / (double)pl.cpd);
     const double invcpd = 1.0 / (double)pl.cpd;
-    float4 *box = box_of(sources4, pl.nsrc_blocks);
+    float4 *box = static_cast<float4 *>(box_after(sources4, pl.nsrc_blocks));
     if (pl.nsink_sets > 0) {
         if (fp64_offsets)
             hipLaunchKernelGGL(k_gather_sinks<true>, dim3(pl.nsink_sets), dim3(256), 0, st, pl, sl.pos[pl.R],
@@ -673,10 +720,8 @@ int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale
 }
 
 template <int SOFT, int S>
-static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const void *sinks4, const void *sources4,
-                            const WorkItem *items, int32_t nitems, int32_t *counter, void *partial_d4,
-                            hipStream_t st) {
-    const float4 *box = box_of(const_cast<void *>(sources4), cx.pl.nsrc_blocks);
+static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const SicView *views, const WorkItem *items,
+                            int32_t nitems, int32_t *counter, hipStream_t st) {
     int blocks_per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_batched<SOFT, S>, 256, 0) != hipSuccess ||
         blocks_per_cu <= 0)
@@ -687,48 +732,42 @@ static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const void 
     const int need = (nitems + 3) / 4;
     if (grid > need) grid = std::max(1, need);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL((k_batched<SOFT, S>), dim3(grid), dim3(256), 0, st, cx, static_cast<const float4 *>(sinks4),
-                       static_cast<const float4 *>(sources4), box, items, nitems, counter,
-                       static_cast<double *>(partial_d4));
+    hipLaunchKernelGGL((k_batched<SOFT, S>), dim3(grid), dim3(256), 0, st, cx, views, items, nitems, counter);
     return (int)hipGetLastError();
 }
 
 template <int SOFT>
-static int launch_batched_soft(const BatchCtx &cx, const TileShape &ts, const void *sinks4, const void *sources4,
-                               const WorkItem *items, int32_t nitems, int32_t *counter, void *partial_d4,
-                               hipStream_t st) {
+static int launch_batched_soft(const BatchCtx &cx, const TileShape &ts, const SicView *views, const WorkItem *items,
+                               int32_t nitems, int32_t *counter, hipStream_t st) {
     switch (ts.sinks_per_lane) {
-        case 1: return launch_batched_s<SOFT, 1>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
-        case 2: return launch_batched_s<SOFT, 2>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
-        case 4: return launch_batched_s<SOFT, 4>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
+        case 1: return launch_batched_s<SOFT, 1>(cx, ts, views, items, nitems, counter, st);
+        case 2: return launch_batched_s<SOFT, 2>(cx, ts, views, items, nitems, counter, st);
+        case 4: return launch_batched_s<SOFT, 4>(cx, ts, views, items, nitems, counter, st);
         default: return (int)hipErrorInvalidValue;
     }
 }
 
-int launch_batched(const DevPlan &pl, int softening, const PairConst &pc, const TileShape &ts, const void *sinks4,
-                   const void *sources4, const WorkItem *items, int32_t nitems, int32_t *counter, void *partial_d4,
-                   void *stream) {
+int launch_batched(const SicView *views, int cpd, int softening, const PairConst &pc, const TileShape &ts,
+                   const WorkItem *items, int32_t nitems, int32_t *counter, void *stream) {
     if (nitems == 0) return 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
     BatchCtx cx;
-    cx.pl = pl;
     cx.pc = pc;
-    cx.invcpd = 1.0 / (double)pl.cpd;
+    cx.invcpd = 1.0 / (double)cpd;
     // The unsoftened branch is exact for r^2 > e^2 = 1/eps; the box test uses
     // a relative margin far above FP32 rounding of the box corners.
     cx.soft2 = (softening == NF_SOFT_SINGLE_SPLINE) ? (float)((1.0 / (double)pc.eps) * 1.01) : 0.f;
+    // Density is nonzero only for r^2 + r2add < b2x; a 1% margin covers the
+    // FP32 rounding of the box gap and of r^2.
+    cx.dens2 = pc.b2x > 0.f ? (float)((double)pc.b2x * 1.01) : -1.f;
     switch (softening) {
-        case NF_SOFT_PLUMMER:
-            return launch_batched_soft<NF_SOFT_PLUMMER>(cx, ts, sinks4, sources4, items, nitems, counter, partial_d4, st);
+        case NF_SOFT_PLUMMER: return launch_batched_soft<NF_SOFT_PLUMMER>(cx, ts, views, items, nitems, counter, st);
         case NF_SOFT_SINGLE_SPLINE:
-            return launch_batched_soft<NF_SOFT_SINGLE_SPLINE>(cx, ts, sinks4, sources4, items, nitems, counter,
-                                                              partial_d4, st);
+            return launch_batched_soft<NF_SOFT_SINGLE_SPLINE>(cx, ts, views, items, nitems, counter, st);
         case NF_SOFT_CUBIC_SPLINE:
-            return launch_batched_soft<NF_SOFT_CUBIC_SPLINE>(cx, ts, sinks4, sources4, items, nitems, counter,
-                                                             partial_d4, st);
+            return launch_batched_soft<NF_SOFT_CUBIC_SPLINE>(cx, ts, views, items, nitems, counter, st);
         case NF_SOFT_CUBIC_PLUMMER:
-            return launch_batched_soft<NF_SOFT_CUBIC_PLUMMER>(cx, ts, sinks4, sources4, items, nitems, counter,
-                                                              partial_d4, st);
+            return launch_batched_soft<NF_SOFT_CUBIC_PLUMMER>(cx, ts, views, items, nitems, counter, st);
         default: return (int)hipErrorInvalidValue;
     }
 }

@@ -15,12 +15,30 @@ int fail(int code, const std::string &msg);
 
 // Work item of the batched kernel: a run of `nsink` sinks starting at
 // `first` within sink pencil `sinkset`.  kind 0: register tile of 64*S sinks,
-// sinks on lanes; kind 1: remainder (< 64*S sinks), sources on lanes.
+// sinks on lanes; kind 2: one-wave tile of up to 64 sinks on the lanes (a
+// pencil's tail; lanes past nsink idle); kind 1: a short tail (fewer than
+// kTailLanes sinks), sources on the lanes.
+constexpr int kTailLanes = 53;  // below this many sinks a tail is cheaper with sources on the lanes
+
 struct WorkItem {
     int32_t sinkset;
     int32_t first;
     int32_t nsink;
-    int32_t kind;
+    int32_t kind_sic;  // kind + 4 * (index of the SIC in its launch group)
+};
+
+// What the batched kernel needs of one SIC of a launch group (device memory).
+struct SicView {
+    const int32_t *sink_set_start;
+    const int32_t *src_set_start;
+    const int32_t *src_set_count;
+    const int32_t *ilist;
+    const void *sinks4;    // padded sink stream, float4
+    const void *sources4;  // padded source stream, float4, followed by 2 float4 boxes per block
+    const void *box4;
+    double *partial;       // 4 doubles per padded sink slot
+    int32_t W, R;
+    int32_t pad[2];
 };
 
 // Device views of the plan arrays of one SIC.
@@ -74,9 +92,13 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void
 int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const void *sinks4,
                  const void *sources4, void *partial4, void *stream);
 int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, void *stream);
-int launch_batched(const DevPlan &pl, int softening, const PairConst &pc, const TileShape &ts,
-                   const void *sinks4, const void *sources4, const WorkItem *items, int32_t nitems,
-                   int32_t *counter, void *partial_d4, void *stream);
+// One persistent launch over the work items of a group of SICs.
+int launch_batched(const SicView *views, int cpd, int softening, const PairConst &pc, const TileShape &ts,
+                   const WorkItem *items, int32_t nitems, int32_t *counter, void *stream);
+// Bounding boxes follow a padded source stream of nsrc_blocks blocks.
+inline void *box_after(void *sources4, int32_t nsrc_blocks) {
+    return static_cast<char *>(sources4) + (size_t)nsrc_blocks * NF_BLOCK_SIZE * 16;
+}
 int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
                      void *stream);
 int device_cu_count();

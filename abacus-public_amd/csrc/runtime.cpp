@@ -18,6 +18,8 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <sstream>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -142,7 +144,7 @@ struct Worker {
     int id = 0;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     DevBuf plan, work, slabs;
     PinBuf pin_plan, pin_acc;
     std::thread th;
@@ -159,6 +161,18 @@ struct Runtime {
     std::condition_variable qcv;
     bool stopping = false;
     TileShape shape{2, 64, 4, 0};
+    int max_group = 8;
+    bool autotune = false;
+    std::mutex tune_m;
+    struct TuneEntry {
+        TileShape shape;
+        std::vector<double> trial_ms;  // per candidate, in candidate order
+        uint64_t interactions = 0;
+        int uses = 0;
+    };
+    std::map<std::string, TuneEntry> tune;  // occupancy-histogram key -> tuned shape
+    hipStream_t shared = nullptr;  // when gpu_streams == 1: the one stream all workers use
+    std::mutex enqueue;            // orders whole task sequences on the shared stream
 };
 
 std::mutex g_rt_mutex;
@@ -166,15 +180,15 @@ Runtime *g_rt = nullptr;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Bytes of plan arrays uploaded per task (plus the batched work list).
+// Where one SIC's plan arrays sit inside a group's plan upload.
 struct PlanLayout {
     size_t o_sink_start, o_sink_count, o_idmax, o_sink_plan, o_src_start, o_src_count, o_src_plan, o_parent,
-        o_ilist, o_yoff, o_items, o_counter, total;
+        o_ilist, o_yoff, total;
 };
 
-PlanLayout plan_layout(const nf_sic &s, int32_t nitems) {
+PlanLayout plan_layout(const nf_sic &s, size_t base) {
     PlanLayout L{};
-    size_t o = 0;
+    size_t o = base;
     auto take = [&](size_t bytes) {
         size_t r = o;
         o = align_up(o + bytes, 256);
@@ -190,18 +204,41 @@ PlanLayout plan_layout(const nf_sic &s, int32_t nitems) {
     L.o_parent = take(4 * (size_t)s.NSinkBlocks);
     L.o_ilist = take(4 * (size_t)s.InteractionCount);
     L.o_yoff = take(4 * (size_t)s.InteractionCount);
-    L.o_items = take(sizeof(WorkItem) * (size_t)std::max(nitems, 1));
-    L.o_counter = take(256);
-    L.total = o;
+    L.total = o - base;
     return L;
 }
 
-// The batched work list: per sink pencil, full register tiles of 64*S sinks
-// and one remainder item, ordered by interaction count (longest first).
-std::vector<WorkItem> build_items(const nf_sic &s, int S) {
-    std::vector<WorkItem> items;
-    std::vector<uint64_t> cost;
-    items.reserve((size_t)s.NSinkSets * 2);
+// Device working set of one SIC: sink stream, source stream + boxes, partials.
+struct WorkLayout {
+    size_t o_sinks, o_src, o_part, total;
+};
+
+WorkLayout work_layout(const nf_sic &s, int path, size_t base) {
+    WorkLayout L{};
+    const size_t psink = (size_t)s.NSinkBlocks * NF_BLOCK_SIZE;
+    const size_t psrc = (size_t)s.NSourceBlocks * NF_BLOCK_SIZE;
+    L.o_sinks = base;
+    L.o_src = align_up(L.o_sinks + 16 * psink, 256);
+    L.o_part = align_up(L.o_src + 16 * psrc + 32 * (size_t)s.NSourceBlocks, 256);
+    const size_t end = align_up(L.o_part + psink * (path == NF_PATH_BATCHED ? 32 : 16), 256);
+    L.total = end - base;
+    return L;
+}
+
+// Device bytes a SIC adds to a group (plan + work + host-mode slabs).
+size_t task_bytes(const nf_sic &s, int path) {
+    size_t b = plan_layout(s, 0).total + work_layout(s, path, 0).total + sizeof(SicView) + 256;
+    b += ((size_t)s.NSinkBlocks + 2 * (size_t)s.NSinkSets) * sizeof(WorkItem);  // at most one item per block + tails
+    if (s.mem == NF_MEM_HOST) {
+        for (int c = 0; c < s.nfwidth; c++) b += align_up(12 * (size_t)s.Nslab[c], 256);
+        b += align_up(16 * (size_t)s.Nslab[s.nfradius], 256);
+    }
+    return b;
+}
+
+// The batched work list of one SIC: per sink pencil, full register tiles of
+// 64*S sinks, then the tail; cost = sinks x sources of the item.
+void build_items(const nf_sic &s, int S, int sic, std::vector<WorkItem> &items, std::vector<uint64_t> &cost) {
     const int tile = NF_BLOCK_SIZE * S;
     const int W = s.nfwidth;
     for (int k = 0; k < s.NSinkSets; k++) {
@@ -211,20 +248,22 @@ std::vector<WorkItem> build_items(const nf_sic &s, int S) {
         for (int y = 0; y < W; y++) nsrc += (uint64_t)s.SourceSetCount[s.SinkSourceInteractionList[W * k + y]];
         int f = 0;
         for (; f + tile <= n; f += tile) {
-            items.push_back(WorkItem{k, f, tile, 0});
+            items.push_back(WorkItem{k, f, tile, 0 + 4 * sic});
             cost.push_back((uint64_t)tile * nsrc);
         }
+        // The tail: one-wave tiles while they keep at least kTailLanes lanes
+        // busy, then the rest with sources on the lanes.
+        while (n - f >= kTailLanes) {
+            const int m = std::min(NF_BLOCK_SIZE, n - f);
+            items.push_back(WorkItem{k, f, m, 2 + 4 * sic});
+            cost.push_back((uint64_t)NF_BLOCK_SIZE * nsrc);
+            f += m;
+        }
         if (f < n) {
-            items.push_back(WorkItem{k, f, n - f, 1});
+            items.push_back(WorkItem{k, f, n - f, 1 + 4 * sic});
             cost.push_back((uint64_t)(n - f) * nsrc);
         }
     }
-    std::vector<size_t> order(items.size());
-    for (size_t i = 0; i < order.size(); i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-    std::vector<WorkItem> out(items.size());
-    for (size_t i = 0; i < order.size(); i++) out[i] = items[order[i]];
-    return out;
 }
 
 int validate(const nf_sic &s) {
@@ -305,177 +344,339 @@ double event_ms(hipEvent_t a, hipEvent_t b) {
     return (double)ms;
 }
 
-// Execute one SIC on worker w (called on the worker's thread, or on the
-// caller's thread for blocking submissions).
-int run_task(Runtime &rt, Worker &w, Task &t) {
+// ---- Autotuner -------------------------------------------------------
+
+// Candidate tile shapes: sinks per lane x persistent waves per CU (0: the
+// occupancy limit).
+const TileShape kCandidates[] = {{1, 64, 4, 0}, {2, 64, 4, 0}, {4, 64, 4, 0}, {2, 64, 4, 8}, {4, 64, 4, 8}};
+constexpr int kNumCandidates = sizeof(kCandidates) / sizeof(kCandidates[0]);
+
+// Histogram of sink-pencil occupancies over log2 bins (0, 1, 2-3, 4-7, ...),
+// as fractions of the group's pencils quantised to 1/16: the tuning key.
+std::string occupancy_key(const std::vector<Task *> &g) {
+    constexpr int kBins = 16;
+    double h[kBins] = {0};
+    double tot = 0;
+    for (Task *t : g)
+        for (int k = 0; k < t->sic.NSinkSets; k++) {
+            const int n = t->sic.SinkSetCount[k];
+            int b = 0;
+            while (b < kBins - 1 && (1 << b) <= n) b++;
+            h[b] += 1;
+            tot += 1;
+        }
+    std::ostringstream os;
+    for (int b = 0; b < kBins; b++) os << (b ? "," : "") << (int)std::lround(16.0 * h[b] / std::max(tot, 1.0));
+    return os.str();
+}
+
+// Execute a group of compatible SICs on worker w: one plan upload, a gather
+// per SIC, one persistent direct-kernel launch over all their work items
+// (batched path; the plain path launches per SIC), a co-add per SIC.
+int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     const auto t0 = clk::now();
-    const nf_sic &s = t.sic;
     NF_HIP(hipSetDevice(w.device));
-    int rc = validate(s);
-    if (rc) return rc;
-    PairConst pc;
-    rc = derive_pair_const(s.softening, s.eps, s.b2, &pc);
-    if (rc) return rc;
     const int path = rt.params.path;
-    const TileShape ts = rt.shape;
-
-    std::vector<WorkItem> items;
-    if (path == NF_PATH_BATCHED) items = build_items(s, ts.sinks_per_lane);
-    const int32_t nitems = (int32_t)items.size();
-    const PlanLayout L = plan_layout(s, nitems);
-
-    // Device working set: sink stream, source stream + boxes, partials.
-    const size_t psink = (size_t)s.NSinkBlocks * NF_BLOCK_SIZE;
-    const size_t psrc = (size_t)s.NSourceBlocks * NF_BLOCK_SIZE;
-    const size_t part_bytes = psink * (path == NF_PATH_BATCHED ? 32 : 16);
-    const size_t o_sinks = 0;
-    const size_t o_src = align_up(o_sinks + 16 * psink, 256);
-    const size_t o_part = align_up(o_src + 16 * psrc + 32 * (size_t)s.NSourceBlocks, 256);
-    const size_t work_bytes = align_up(o_part + part_bytes, 256);
-
-    // Host mode: the W slabs and the sink acceleration slab on the device.
-    size_t slab_off[NF_MAX_WIDTH] = {0};
-    size_t slab_bytes = 0, o_acc = 0;
-    const uint64_t nsink_slab = s.Nslab[s.nfradius];
-    if (s.mem == NF_MEM_HOST) {
-        for (int c = 0; c < s.nfwidth; c++) {
-            slab_off[c] = slab_bytes;
-            slab_bytes = align_up(slab_bytes + 12 * (size_t)s.Nslab[c], 256);
+    TileShape ts = rt.shape;
+    const int n = (int)g.size();
+    // Autotuning: a known occupancy histogram gets its tuned shape; a new one
+    // is tuned on this group's launch (every candidate computes the same
+    // partials; the last launch's are co-added).
+    std::string tkey;
+    bool trial = false;
+    if (path == NF_PATH_BATCHED && rt.autotune) {
+        tkey = occupancy_key(g);
+        std::lock_guard<std::mutex> lk(rt.tune_m);
+        auto it = rt.tune.find(tkey);
+        if (it != rt.tune.end()) {
+            ts = it->second.shape;
+            it->second.uses++;
+        } else {
+            trial = true;
         }
-        o_acc = slab_bytes;
-        slab_bytes = align_up(slab_bytes + 16 * (size_t)nsink_slab, 256);
     }
-    if (L.total + work_bytes + slab_bytes > rt.max_buffer_bytes)
-        return fail(NF_ERR_CAPACITY, "SIC needs " + std::to_string(L.total + work_bytes + slab_bytes) +
+    const nf_sic &s0 = g[0]->sic;
+    PairConst pc;
+    int rc = derive_pair_const(s0.softening, s0.eps, s0.b2, &pc);
+    if (rc) return rc;
+
+    // Work list over the whole group, longest items first: one list per
+    // distinct sinks-per-lane value to be launched (several while tuning).
+    auto make_items = [&](int S) {
+        std::vector<WorkItem> raw, out;
+        std::vector<uint64_t> cost;
+        for (int i = 0; i < n; i++) build_items(g[i]->sic, S, i, raw, cost);
+        std::vector<size_t> order(raw.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        out.resize(raw.size());
+        for (size_t i = 0; i < order.size(); i++) out[i] = raw[order[i]];
+        return out;
+    };
+    std::vector<int> Svals;
+    if (path == NF_PATH_BATCHED) {
+        if (trial)
+            for (const TileShape &c : kCandidates)
+                if (std::find(Svals.begin(), Svals.end(), c.sinks_per_lane) == Svals.end()) Svals.push_back(c.sinks_per_lane);
+        if (std::find(Svals.begin(), Svals.end(), ts.sinks_per_lane) == Svals.end()) Svals.push_back(ts.sinks_per_lane);
+    }
+    std::vector<std::vector<WorkItem>> itemsets;
+    for (int S : Svals) itemsets.push_back(make_items(S));
+    auto items_of = [&](int S) -> int {
+        for (size_t k = 0; k < Svals.size(); k++)
+            if (Svals[k] == S) return (int)k;
+        return 0;
+    };
+    size_t items_total = 0;
+    for (auto &v : itemsets) items_total += align_up(sizeof(WorkItem) * std::max<size_t>(v.size(), 1), 256);
+
+    // Plan upload: [views][items][counter][plan arrays of each SIC].
+    const size_t o_views = 0;
+    const size_t o_items = align_up(o_views + sizeof(SicView) * (size_t)n, 256);
+    std::vector<size_t> o_itemset(itemsets.size());
+    {
+        size_t o = o_items;
+        for (size_t k = 0; k < itemsets.size(); k++) {
+            o_itemset[k] = o;
+            o += align_up(sizeof(WorkItem) * std::max<size_t>(itemsets[k].size(), 1), 256);
+        }
+    }
+    const size_t o_counter = align_up(o_items + std::max<size_t>(items_total, 256), 256);
+    size_t plan_total = align_up(o_counter + 256, 256);
+    std::vector<PlanLayout> PL(n);
+    for (int i = 0; i < n; i++) {
+        PL[i] = plan_layout(g[i]->sic, plan_total);
+        plan_total += PL[i].total;
+    }
+    // Device working set of each SIC.
+    std::vector<WorkLayout> WL(n);
+    size_t work_total = 0;
+    for (int i = 0; i < n; i++) {
+        WL[i] = work_layout(g[i]->sic, path, work_total);
+        work_total += WL[i].total;
+    }
+    // Host mode: every distinct host slab once, plus an acceleration slab per SIC.
+    std::vector<const float *> hslab;
+    std::vector<uint64_t> hslab_n;
+    std::vector<size_t> hslab_off;
+    std::vector<size_t> acc_off(n, 0);
+    size_t slab_total = 0;
+    if (s0.mem == NF_MEM_HOST) {
+        for (int i = 0; i < n; i++) {
+            const nf_sic &s = g[i]->sic;
+            for (int c = 0; c < s.nfwidth; c++) {
+                if (std::find(hslab.begin(), hslab.end(), s.SourcePosSlab[c]) != hslab.end()) continue;
+                hslab.push_back(s.SourcePosSlab[c]);
+                hslab_n.push_back(s.Nslab[c]);
+                hslab_off.push_back(slab_total);
+                slab_total = align_up(slab_total + 12 * (size_t)s.Nslab[c], 256);
+            }
+            acc_off[i] = slab_total;
+            slab_total = align_up(slab_total + 16 * (size_t)s.Nslab[s.nfradius], 256);
+        }
+    }
+    if (plan_total + work_total + slab_total > rt.max_buffer_bytes)
+        return fail(NF_ERR_CAPACITY, "SIC group needs " + std::to_string(plan_total + work_total + slab_total) +
                                          " device bytes, buffer holds " + std::to_string(rt.max_buffer_bytes));
-    if ((rc = w.plan.ensure(L.total)) || (rc = w.work.ensure(work_bytes)) || (rc = w.pin_plan.ensure(L.total)))
+    if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) || (rc = w.pin_plan.ensure(plan_total)))
         return rc;
-    if (s.mem == NF_MEM_HOST && ((rc = w.slabs.ensure(slab_bytes)) || (rc = w.pin_acc.ensure(16 * nsink_slab))))
-        return rc;
+    if (s0.mem == NF_MEM_HOST && (rc = w.slabs.ensure(slab_total))) return rc;
 
-    // ---- Fill: plan arrays through pinned staging (and slabs in host mode).
+    // ---- Fill the pinned plan image.
     char *hp = static_cast<char *>(w.pin_plan.p);
-    std::memcpy(hp + L.o_sink_start, s.SinkSetStart, 4 * (size_t)s.NSinkSets);
-    std::memcpy(hp + L.o_sink_count, s.SinkSetCount, 4 * (size_t)s.NSinkSets);
-    std::memcpy(hp + L.o_idmax, s.SinkSetIdMax, 4 * (size_t)s.NSinkSets);
-    std::memcpy(hp + L.o_sink_plan, s.SinkPlan, sizeof(nf_sink_plan) * (size_t)s.NSinkSets);
-    std::memcpy(hp + L.o_src_start, s.SourceSetStart, 4 * (size_t)s.NSourceSets);
-    std::memcpy(hp + L.o_src_count, s.SourceSetCount, 4 * (size_t)s.NSourceSets);
-    std::memcpy(hp + L.o_src_plan, s.SourcePlan, sizeof(nf_source_plan) * (size_t)s.NSourceSets);
-    if (s.NSinkBlocks) std::memcpy(hp + L.o_parent, s.SinkBlockParentPencil, 4 * (size_t)s.NSinkBlocks);
-    std::memcpy(hp + L.o_ilist, s.SinkSourceInteractionList, 4 * (size_t)s.InteractionCount);
-    std::memcpy(hp + L.o_yoff, s.SinkSourceYOffset, 4 * (size_t)s.InteractionCount);
-    if (nitems) std::memcpy(hp + L.o_items, items.data(), sizeof(WorkItem) * (size_t)nitems);
-    NF_HIP(hipEventRecord(w.ev[0], w.stream));
-    NF_HIP(hipMemcpyAsync(w.plan.p, hp, L.total, hipMemcpyHostToDevice, w.stream));
-    uint64_t to_dev = L.total, from_dev = 0;
-
-    DevSlabs sl{};
-    float *acc_dev = nullptr;
-    if (s.mem == NF_MEM_HOST) {
-        char *db = static_cast<char *>(w.slabs.p);
-        for (int c = 0; c < s.nfwidth; c++) {
-            NF_HIP(hipMemcpyAsync(db + slab_off[c], s.SourcePosSlab[c], 12 * (size_t)s.Nslab[c],
-                                  hipMemcpyHostToDevice, w.stream));
-            to_dev += 12 * s.Nslab[c];
-            sl.pos[c] = reinterpret_cast<const float *>(db + slab_off[c]);
-            sl.n[c] = s.Nslab[c];
-        }
-        acc_dev = reinterpret_cast<float *>(db + o_acc);
-    } else {
-        for (int c = 0; c < s.nfwidth; c++) {
-            sl.pos[c] = s.SourcePosSlab[c];
-            sl.n[c] = s.Nslab[c];
-        }
-        // The sink slab is one of the source slabs (c = R); use the SIC's own pointer.
-        sl.pos[s.nfradius] = s.SinkPosSlab;
-        acc_dev = s.SinkAccSlab;
-    }
-
-    const char *pd = static_cast<const char *>(w.plan.p);
-    DevPlan pl{};
-    pl.sink_set_start = reinterpret_cast<const int32_t *>(pd + L.o_sink_start);
-    pl.sink_set_count = reinterpret_cast<const int32_t *>(pd + L.o_sink_count);
-    pl.sink_idmax = reinterpret_cast<const uint32_t *>(pd + L.o_idmax);
-    pl.sink_plan = reinterpret_cast<const nf_sink_plan *>(pd + L.o_sink_plan);
-    pl.src_set_start = reinterpret_cast<const int32_t *>(pd + L.o_src_start);
-    pl.src_set_count = reinterpret_cast<const int32_t *>(pd + L.o_src_count);
-    pl.src_plan = reinterpret_cast<const nf_source_plan *>(pd + L.o_src_plan);
-    pl.parent = reinterpret_cast<const int32_t *>(pd + L.o_parent);
-    pl.ilist = reinterpret_cast<const int32_t *>(pd + L.o_ilist);
-    pl.yoff = reinterpret_cast<const float *>(pd + L.o_yoff);
-    pl.nsink_sets = s.NSinkSets;
-    pl.nsrc_sets = s.NSourceSets;
-    pl.nsink_blocks = s.NSinkBlocks;
-    pl.nsrc_blocks = s.NSourceBlocks;
-    pl.W = s.nfwidth;
-    pl.R = s.nfradius;
-    pl.cpd = s.cpd;
-    pl.Nk = s.Nk;
-    pl.j_width = s.j_width;
-    pl.j_low = s.j_low;
-
+    char *pd = static_cast<char *>(w.plan.p);
     char *wb = static_cast<char *>(w.work.p);
-    void *sinks4 = wb + o_sinks, *src4 = wb + o_src, *part = wb + o_part;
-    // Accelerations of sink rows [j_low, j_high) are written through the
-    // first cell of the first row: the co-add indexes the slab by cell start.
+    char *db = static_cast<char *>(w.slabs.p);
+    std::vector<DevPlan> plans(n);
+    std::vector<DevSlabs> slabs(n);
+    std::vector<float *> accd(n);
+    SicView *views = reinterpret_cast<SicView *>(hp + o_views);
+    for (int i = 0; i < n; i++) {
+        const nf_sic &s = g[i]->sic;
+        const PlanLayout &L = PL[i];
+        std::memcpy(hp + L.o_sink_start, s.SinkSetStart, 4 * (size_t)s.NSinkSets);
+        std::memcpy(hp + L.o_sink_count, s.SinkSetCount, 4 * (size_t)s.NSinkSets);
+        std::memcpy(hp + L.o_idmax, s.SinkSetIdMax, 4 * (size_t)s.NSinkSets);
+        std::memcpy(hp + L.o_sink_plan, s.SinkPlan, sizeof(nf_sink_plan) * (size_t)s.NSinkSets);
+        std::memcpy(hp + L.o_src_start, s.SourceSetStart, 4 * (size_t)s.NSourceSets);
+        std::memcpy(hp + L.o_src_count, s.SourceSetCount, 4 * (size_t)s.NSourceSets);
+        std::memcpy(hp + L.o_src_plan, s.SourcePlan, sizeof(nf_source_plan) * (size_t)s.NSourceSets);
+        if (s.NSinkBlocks) std::memcpy(hp + L.o_parent, s.SinkBlockParentPencil, 4 * (size_t)s.NSinkBlocks);
+        std::memcpy(hp + L.o_ilist, s.SinkSourceInteractionList, 4 * (size_t)s.InteractionCount);
+        std::memcpy(hp + L.o_yoff, s.SinkSourceYOffset, 4 * (size_t)s.InteractionCount);
+
+        DevPlan &pl = plans[i];
+        pl.sink_set_start = reinterpret_cast<const int32_t *>(pd + L.o_sink_start);
+        pl.sink_set_count = reinterpret_cast<const int32_t *>(pd + L.o_sink_count);
+        pl.sink_idmax = reinterpret_cast<const uint32_t *>(pd + L.o_idmax);
+        pl.sink_plan = reinterpret_cast<const nf_sink_plan *>(pd + L.o_sink_plan);
+        pl.src_set_start = reinterpret_cast<const int32_t *>(pd + L.o_src_start);
+        pl.src_set_count = reinterpret_cast<const int32_t *>(pd + L.o_src_count);
+        pl.src_plan = reinterpret_cast<const nf_source_plan *>(pd + L.o_src_plan);
+        pl.parent = reinterpret_cast<const int32_t *>(pd + L.o_parent);
+        pl.ilist = reinterpret_cast<const int32_t *>(pd + L.o_ilist);
+        pl.yoff = reinterpret_cast<const float *>(pd + L.o_yoff);
+        pl.nsink_sets = s.NSinkSets;
+        pl.nsrc_sets = s.NSourceSets;
+        pl.nsink_blocks = s.NSinkBlocks;
+        pl.nsrc_blocks = s.NSourceBlocks;
+        pl.W = s.nfwidth;
+        pl.R = s.nfradius;
+        pl.cpd = s.cpd;
+        pl.Nk = s.Nk;
+        pl.j_width = s.j_width;
+        pl.j_low = s.j_low;
+
+        DevSlabs &sl = slabs[i];
+        for (int c = 0; c < s.nfwidth; c++) {
+            sl.n[c] = s.Nslab[c];
+            if (s.mem == NF_MEM_HOST) {
+                const size_t k = std::find(hslab.begin(), hslab.end(), s.SourcePosSlab[c]) - hslab.begin();
+                sl.pos[c] = reinterpret_cast<const float *>(db + hslab_off[k]);
+            } else {
+                sl.pos[c] = s.SourcePosSlab[c];
+            }
+        }
+        if (s.mem == NF_MEM_DEVICE) sl.pos[s.nfradius] = s.SinkPosSlab;
+        accd[i] = s.mem == NF_MEM_HOST ? reinterpret_cast<float *>(db + acc_off[i]) : s.SinkAccSlab;
+
+        SicView &v = views[i];
+        std::memset(&v, 0, sizeof(v));
+        v.sink_set_start = pl.sink_set_start;
+        v.src_set_start = pl.src_set_start;
+        v.src_set_count = pl.src_set_count;
+        v.ilist = pl.ilist;
+        v.sinks4 = wb + WL[i].o_sinks;
+        v.sources4 = wb + WL[i].o_src;
+        v.box4 = box_after(wb + WL[i].o_src, s.NSourceBlocks);
+        v.partial = reinterpret_cast<double *>(wb + WL[i].o_part);
+        v.W = s.nfwidth;
+        v.R = s.nfradius;
+    }
+    for (size_t k = 0; k < itemsets.size(); k++)
+        if (!itemsets[k].empty())
+            std::memcpy(hp + o_itemset[k], itemsets[k].data(), sizeof(WorkItem) * itemsets[k].size());
     const auto t_fill = clk::now();
 
-    NF_HIP(hipEventRecord(w.ev[1], w.stream));
-    int e = launch_gather(pl, sl, path == NF_PATH_BATCHED, sinks4, src4, w.stream);
+    // ---- Enqueue the group as one uninterrupted sequence.
+    hipStream_t stream = rt.shared ? rt.shared : w.stream;
+    std::unique_lock<std::mutex> enqueue_lock(rt.enqueue, std::defer_lock);
+    if (rt.shared) enqueue_lock.lock();
+    uint64_t to_dev = plan_total, from_dev = 0;
+    NF_HIP(hipEventRecord(w.ev[0], stream));
+    NF_HIP(hipMemcpyAsync(w.plan.p, hp, plan_total, hipMemcpyHostToDevice, stream));
+    for (size_t k = 0; k < hslab.size(); k++) {
+        NF_HIP(hipMemcpyAsync(db + hslab_off[k], hslab[k], 12 * (size_t)hslab_n[k], hipMemcpyHostToDevice, stream));
+        to_dev += 12 * hslab_n[k];
+    }
+    NF_HIP(hipEventRecord(w.ev[1], stream));
+    int e = 0;
+    for (int i = 0; i < n && !e; i++)
+        e = launch_gather(plans[i], slabs[i], path == NF_PATH_BATCHED, wb + WL[i].o_sinks, wb + WL[i].o_src, stream);
     if (e) return fail(NF_ERR_HIP, std::string("gather launch: ") + hipGetErrorString((hipError_t)e));
-    NF_HIP(hipEventRecord(w.ev[2], w.stream));
+    NF_HIP(hipEventRecord(w.ev[2], stream));
+    std::vector<hipEvent_t> tev;
     if (path == NF_PATH_BATCHED) {
-        int32_t *counter = reinterpret_cast<int32_t *>(static_cast<char *>(w.plan.p) + L.o_counter);
-        const WorkItem *ditems = reinterpret_cast<const WorkItem *>(pd + L.o_items);
-        e = launch_batched(pl, s.softening, pc, ts, sinks4, src4, ditems, nitems, counter, part, w.stream);
+        int32_t *counter = reinterpret_cast<int32_t *>(pd + o_counter);
+        auto launch = [&](const TileShape &shape) {
+            const int k = items_of(shape.sinks_per_lane);
+            return launch_batched(reinterpret_cast<const SicView *>(pd + o_views), s0.cpd, s0.softening, pc, shape,
+                                  reinterpret_cast<const WorkItem *>(pd + o_itemset[k]),
+                                  (int32_t)itemsets[k].size(), counter, stream);
+        };
+        if (trial) {
+            // Time each candidate; the chosen shape runs last so its partials
+            // (identical up to summation order) are the ones co-added.
+            tev.resize(kNumCandidates + 1);
+            for (auto &ev : tev) NF_HIP(hipEventCreate(&ev));
+            NF_HIP(hipEventRecord(tev[0], stream));
+            for (int c = 0; c < kNumCandidates && !e; c++) {
+                e = launch(kCandidates[c]);
+                NF_HIP(hipEventRecord(tev[c + 1], stream));
+            }
+        }
+        if (!e) {
+            NF_HIP(hipEventRecord(w.ev[2], stream));
+            e = launch(ts);
+        }
     } else {
-        e = launch_plain(pl, s.softening, pc, sinks4, src4, part, w.stream);
+        for (int i = 0; i < n && !e; i++)
+            e = launch_plain(plans[i], s0.softening, pc, wb + WL[i].o_sinks, wb + WL[i].o_src, wb + WL[i].o_part,
+                             stream);
     }
     if (e) return fail(NF_ERR_HIP, std::string("direct kernel launch: ") + hipGetErrorString((hipError_t)e));
-    NF_HIP(hipEventRecord(w.ev[3], w.stream));
-    if (path == NF_PATH_BATCHED) e = launch_coadd_f64(pl, part, pc.out_scale, acc_dev, w.stream);
-    else e = launch_coadd_plain(pl, part, acc_dev, w.stream);
+    NF_HIP(hipEventRecord(w.ev[3], stream));
+    for (int i = 0; i < n && !e; i++) {
+        if (path == NF_PATH_BATCHED) e = launch_coadd_f64(plans[i], wb + WL[i].o_part, pc.out_scale, accd[i], stream);
+        else e = launch_coadd_plain(plans[i], wb + WL[i].o_part, accd[i], stream);
+    }
     if (e) return fail(NF_ERR_HIP, std::string("co-add launch: ") + hipGetErrorString((hipError_t)e));
-
-    hipEvent_t evd = nullptr;
-    NF_HIP(hipEventCreate(&evd));
-    NF_HIP(hipEventRecord(evd, w.stream));
-    NF_HIP(hipEventSynchronize(evd));
-    const double kern_ms = event_ms(w.ev[1], evd);
+    NF_HIP(hipEventRecord(w.ev[4], stream));
+    if (enqueue_lock.owns_lock()) enqueue_lock.unlock();
+    NF_HIP(hipEventSynchronize(w.ev[4]));
+    const double kern_ms = event_ms(w.ev[1], w.ev[4]);
     const double direct_ms = event_ms(w.ev[2], w.ev[3]);
-    (void)hipEventDestroy(evd);
+    if (trial) {
+        Runtime::TuneEntry te;
+        int best = 0;
+        for (int c = 0; c < kNumCandidates; c++) {
+            te.trial_ms.push_back(event_ms(tev[c], tev[c + 1]));
+            if (te.trial_ms[c] < te.trial_ms[best]) best = c;
+        }
+        for (auto &ev : tev) (void)hipEventDestroy(ev);
+        te.shape = kCandidates[best];
+        for (Task *t : g) te.interactions += t->sic.DirectTotal;
+        te.uses = 1;
+        std::lock_guard<std::mutex> lk(rt.tune_m);
+        rt.tune.emplace(tkey, te);
+    }
 
+    // ---- Host mode: the SICs' sink rows back into their AccSlabs.
     double copyback = 0.0;
-    if (s.mem == NF_MEM_HOST) {
-        // Rows [j_low, j_high) are contiguous in the slab (cells are y-major).
-        const uint64_t r0 = s.SinkPlan[0].cell[0].start;
-        const nf_sink_plan &last = s.SinkPlan[(size_t)(s.j_width - 1) * s.Nk + (s.Nk - 1)];
-        // pencil (j_high-1, cpd-1) holds cell cpd-1 of the last row at c = 0
-        const uint64_t r1 = (uint64_t)last.cell[0].start + (uint64_t)last.cell[0].N;
+    if (s0.mem == NF_MEM_HOST) {
         const auto tc = clk::now();
-        if (r1 > r0) {
-            NF_HIP(hipMemcpyAsync(static_cast<char *>(w.pin_acc.p), acc_dev + 4 * r0, 16 * (r1 - r0),
-                                  hipMemcpyDeviceToHost, w.stream));
-            NF_HIP(hipStreamSynchronize(w.stream));
+        for (int i = 0; i < n; i++) {
+            const nf_sic &s = g[i]->sic;
+            // Rows [j_low, j_high) are contiguous (cells are y-major): from cell
+            // (j_low, 0) -- pencil 0's first cell -- to the end of cell
+            // (j_high-1, cpd-1) -- the first cell of the last pencil.
+            const uint64_t r0 = s.SinkPlan[0].cell[0].start;
+            const nf_sink_plan &last = s.SinkPlan[(size_t)(s.j_width - 1) * s.Nk + (s.Nk - 1)];
+            const uint64_t r1 = (uint64_t)last.cell[0].start + (uint64_t)last.cell[0].N;
+            if (r1 <= r0) continue;
+            if ((rc = w.pin_acc.ensure(16 * (r1 - r0)))) return rc;
+            if (rt.shared) enqueue_lock.lock();
+            NF_HIP(hipMemcpyAsync(w.pin_acc.p, accd[i] + 4 * r0, 16 * (r1 - r0), hipMemcpyDeviceToHost, stream));
+            NF_HIP(hipEventRecord(w.ev[0], stream));
+            if (enqueue_lock.owns_lock()) enqueue_lock.unlock();
+            NF_HIP(hipEventSynchronize(w.ev[0]));
             std::memcpy(s.SinkAccSlab + 4 * r0, w.pin_acc.p, 16 * (r1 - r0));
             from_dev += 16 * (r1 - r0);
         }
         copyback = std::chrono::duration<double>(clk::now() - tc).count();
-    } else {
-        NF_HIP(hipStreamSynchronize(w.stream));
     }
-    t.stats.device_thread_s = std::chrono::duration<double>(clk::now() - t0).count();
-    t.stats.fill_s = std::chrono::duration<double>(t_fill - t0).count();
-    t.stats.kernel_s = kern_ms * 1e-3;
-    t.stats.direct_kernel_s = direct_ms * 1e-3;
-    t.stats.copyback_s = copyback;
-    t.stats.bytes_to_device = to_dev;
-    t.stats.bytes_from_device = from_dev;
-    t.stats.direct_total = s.DirectTotal;
-    t.stats.padded_direct_total = s.PaddedDirectTotal;
-    t.stats.assigned_buffer = w.id;
-    t.stats.path = path;
+    uint64_t gdi = 0;
+    for (int i = 0; i < n; i++) gdi += g[i]->sic.DirectTotal;
+    const double wall = std::chrono::duration<double>(clk::now() - t0).count();
+    for (int i = 0; i < n; i++) {
+        const nf_sic &s = g[i]->sic;
+        const double share = gdi ? (double)s.DirectTotal / (double)gdi : 1.0 / n;
+        nf_task_stats &st = g[i]->stats;
+        st.device_thread_s = wall;
+        st.fill_s = std::chrono::duration<double>(t_fill - t0).count();
+        st.kernel_s = kern_ms * 1e-3 * share;
+        st.direct_kernel_s = direct_ms * 1e-3 * share;
+        st.copyback_s = copyback * share;
+        st.bytes_to_device = (uint64_t)((double)to_dev * share);
+        st.bytes_from_device = (uint64_t)((double)from_dev * share);
+        st.direct_total = s.DirectTotal;
+        st.padded_direct_total = s.PaddedDirectTotal;
+        st.assigned_buffer = w.id;
+        st.path = path;
+        st.group_size = n;
+        st.group_direct_kernel_s = direct_ms * 1e-3;
+        st.group_direct_total = gdi;
+    }
     return NF_OK;
 }
 
@@ -488,18 +689,40 @@ void finish(Task &t, int rc) {
     t.cv.notify_all();
 }
 
+// SICs that may share one launch: same softening constants and memory kind.
+bool compatible(const nf_sic &a, const nf_sic &b) {
+    return a.softening == b.softening && a.eps == b.eps && a.b2 == b.b2 && a.mem == b.mem && a.cpd == b.cpd;
+}
+
 void worker_main(Runtime *rt, Worker *w) {
     for (;;) {
-        Task *t = nullptr;
+        std::vector<Task *> g;
         {
             std::unique_lock<std::mutex> lk(rt->qm);
             rt->qcv.wait(lk, [&] { return rt->stopping || !rt->queue.empty(); });
             if (rt->queue.empty()) return;  // stopping and drained
-            t = rt->queue.front();
+            g.push_back(rt->queue.front());
             rt->queue.pop_front();
+            size_t bytes = task_bytes(g[0]->sic, rt->params.path);
+            while (rt->params.path == NF_PATH_BATCHED && (int)g.size() < rt->max_group && !rt->queue.empty() &&
+                   compatible(g[0]->sic, rt->queue.front()->sic)) {
+                const size_t b = task_bytes(rt->queue.front()->sic, rt->params.path);
+                if (bytes + b > rt->max_buffer_bytes) break;
+                bytes += b;
+                g.push_back(rt->queue.front());
+                rt->queue.pop_front();
+            }
         }
-        int rc = run_task(*rt, *w, *t);
-        finish(*t, rc);
+        // Invalid descriptors fail alone; the rest of the group runs.
+        std::vector<Task *> ok;
+        for (Task *t : g) {
+            const int rc = validate(t->sic);
+            if (rc) finish(*t, rc);
+            else ok.push_back(t);
+        }
+        if (ok.empty()) continue;
+        const int rc = run_group(*rt, *w, ok);
+        for (Task *t : ok) finish(*t, rc);
     }
 }
 
@@ -575,10 +798,14 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
     rt->params = *p;
     rt->ngpu = 1;
     rt->max_buffer_bytes = p->max_buffer_bytes;
+    if (p->max_group < 0 || p->max_group > 4096) return fail(NF_ERR_INVALID, "max_group must be 0..4096");
+    rt->max_group = p->max_group ? p->max_group : 8;
     if (p->sinks_per_lane != 0) {
         if (p->sinks_per_lane != 1 && p->sinks_per_lane != 2 && p->sinks_per_lane != 4)
             return fail(NF_ERR_INVALID, "sinks_per_lane must be 0, 1, 2 or 4");
         rt->shape.sinks_per_lane = p->sinks_per_lane;
+    } else {
+        rt->autotune = true;  // the shape comes from the autotuner per occupancy histogram
     }
     if (p->source_tile != 0 && p->source_tile != 64) return fail(NF_ERR_INVALID, "source_tile must be 0 or 64");
     // Capacity per SIC, from the bytes a sink block brings with it (the
@@ -604,6 +831,16 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
         }
         rt->workers.push_back(std::move(w));
     }
+    if (p->gpu_streams == 1) {
+        hipError_t e = hipStreamCreateWithFlags(&rt->shared, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (auto &x : rt->workers) destroy_worker(*x);
+            return fail(NF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+    } else if (p->gpu_streams != 0) {
+        for (auto &x : rt->workers) destroy_worker(*x);
+        return fail(NF_ERR_INVALID, "gpu_streams must be 0 or 1");
+    }
     for (auto &w : rt->workers) w->th = std::thread(worker_main, rt.get(), w.get());
     g_rt = rt.release();
     return NF_OK;
@@ -620,6 +857,10 @@ extern "C" int nf_reset(void) {
     for (auto &w : g_rt->workers)
         if (w->th.joinable()) w->th.join();
     for (auto &w : g_rt->workers) destroy_worker(*w);
+    if (g_rt->shared) {
+        (void)hipStreamSynchronize(g_rt->shared);
+        (void)hipStreamDestroy(g_rt->shared);
+    }
     delete g_rt;
     g_rt = nullptr;
     return NF_OK;
@@ -677,4 +918,62 @@ extern "C" int nf_release(nf_task *task) {
     if (t->state.load(std::memory_order_acquire) == 0) return fail(NF_ERR_STATE, "task still running");
     delete t;
     return NF_OK;
+}
+
+extern "C" int nf_device_alloc(uint64_t bytes, void **ptr) {
+    if (!ptr) return fail(NF_ERR_INVALID, "null argument");
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(NF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return NF_OK;
+}
+
+extern "C" int nf_device_free(void *ptr) {
+    if (ptr) NF_HIP(hipFree(ptr));
+    return NF_OK;
+}
+
+extern "C" int nf_device_copy(void *dst, const void *src, uint64_t bytes, int32_t kind) {
+    if (bytes == 0) return NF_OK;
+    if (!dst || !src) return fail(NF_ERR_INVALID, "null argument");
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                      : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2) return fail(NF_ERR_INVALID, "bad copy kind");
+    NF_HIP(hipMemcpy(dst, src, bytes, k));
+    return NF_OK;
+}
+
+extern "C" int nf_device_synchronize(void) {
+    NF_HIP(hipDeviceSynchronize());
+    return NF_OK;
+}
+
+extern "C" int nf_autotune_table(char *buf, size_t len) {
+    std::lock_guard<std::mutex> lk(g_rt_mutex);
+    if (!g_rt) return fail(NF_ERR_STATE, "nf_autotune_table before nf_setup");
+    std::ostringstream os;
+    os << "[";
+    {
+        std::lock_guard<std::mutex> tk(g_rt->tune_m);
+        bool first = true;
+        for (const auto &kv : g_rt->tune) {
+            const auto &e = kv.second;
+            os << (first ? "" : ",") << "{\"occupancy_hist_log2\":[" << kv.first << "],\"sinks_per_lane\":"
+               << e.shape.sinks_per_lane << ",\"waves_per_cu\":" << e.shape.waves_per_cu << ",\"uses\":" << e.uses
+               << ",\"interactions\":" << e.interactions << ",\"trial_ms\":[";
+            for (size_t c = 0; c < e.trial_ms.size(); c++) os << (c ? "," : "") << e.trial_ms[c];
+            os << "],\"candidates\":[";
+            for (int c = 0; c < kNumCandidates; c++)
+                os << (c ? "," : "") << "[" << kCandidates[c].sinks_per_lane << "," << kCandidates[c].waves_per_cu << "]";
+            os << "]}";
+            first = false;
+        }
+    }
+    os << "]";
+    const std::string out = os.str();
+    if (buf && len) std::snprintf(buf, len, "%s", out.c_str());
+    return (int)out.size();
 }

@@ -70,7 +70,8 @@ class nf_task_stats(C.Structure):
         ("direct_kernel_s", C.c_double), ("copyback_s", C.c_double),
         ("bytes_to_device", C.c_uint64), ("bytes_from_device", C.c_uint64),
         ("direct_total", C.c_uint64), ("padded_direct_total", C.c_uint64),
-        ("assigned_buffer", C.c_int32), ("path", C.c_int32),
+        ("assigned_buffer", C.c_int32), ("path", C.c_int32), ("group_size", C.c_int32),
+        ("group_direct_kernel_s", C.c_double), ("group_direct_total", C.c_uint64),
     ]
 
 
@@ -78,7 +79,8 @@ class nf_setup_params(C.Structure):
     _fields_ = [
         ("cpd", C.c_int32), ("max_buffer_bytes", C.c_uint64), ("n_gpu", C.c_int32),
         ("buffers_per_device", C.c_int32), ("use_pinned", C.c_int32), ("path", C.c_int32),
-        ("sinks_per_lane", C.c_int32), ("source_tile", C.c_int32),
+        ("sinks_per_lane", C.c_int32), ("source_tile", C.c_int32), ("gpu_streams", C.c_int32),
+        ("max_group", C.c_int32),
     ]
 
 
@@ -103,6 +105,11 @@ def load_library(path=LIB_PATH):
     lib.nf_plan_sic.argtypes = [I32, I32, I32, I32, I32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                 C.POINTER(U64), C.c_float, C.c_float, I32, C.c_void_p, U64,
                                 C.POINTER(nf_sic)]
+    lib.nf_device_alloc.argtypes = [U64, C.POINTER(C.c_void_p)]
+    lib.nf_device_free.argtypes = [C.c_void_p]
+    lib.nf_device_copy.argtypes = [C.c_void_p, C.c_void_p, U64, I32]
+    lib.nf_device_synchronize.argtypes = []
+    lib.nf_autotune_table.argtypes = [C.c_char_p, C.c_size_t]
     return lib
 
 
@@ -131,6 +138,49 @@ def _view(ptr, n, dtype):
         return np.zeros(0, dtype)
     buf = (C.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
     return np.frombuffer(buf, dtype=dtype, count=n)
+
+
+class DeviceArray:
+    """A device allocation holding a copy of a numpy array's bytes (HBM-resident
+    slab or acceleration slab), through the module's own HIP runtime."""
+
+    def __init__(self, nbytes=0, host=None):
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            nbytes = host.nbytes
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().nf_device_alloc(self.nbytes, C.byref(p)), "nf_device_alloc")
+        self.ptr = p.value
+        if host is not None and self.nbytes:
+            check(lib().nf_device_copy(self.ptr, host.ctypes.data, self.nbytes, 0), "nf_device_copy")
+
+    def to_host(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes == self.nbytes
+        if self.nbytes:
+            check(lib().nf_device_copy(out.ctypes.data, self.ptr, self.nbytes, 1), "nf_device_copy")
+        return out
+
+    def zero(self):
+        z = np.zeros(self.nbytes, np.uint8)
+        if self.nbytes:
+            check(lib().nf_device_copy(self.ptr, z.ctypes.data, self.nbytes, 0), "nf_device_copy")
+
+    def free(self):
+        if self.ptr:
+            check(lib().nf_device_free(self.ptr), "nf_device_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    check(lib().nf_device_synchronize(), "nf_device_synchronize")
 
 
 class SetInteractionCollection:
@@ -221,7 +271,7 @@ class NearFieldDriver:
 
     def __init__(self, cpd, softening_length, b2, nfradius=2, softening="single_spline",
                  path="batched", buffers_per_device=3, max_buffer_bytes=None,
-                 sinks_per_lane=0):
+                 sinks_per_lane=0, gpu_streams=0, max_group=0):
         from synth import eps_param
         if NearFieldDriver._active is not None:
             raise NFError("one NearFieldDriver per process (the GPU module is a singleton)")
@@ -237,7 +287,8 @@ class NearFieldDriver:
             max_buffer_bytes = int(free.value) // max(1, buffers_per_device)
         p = nf_setup_params(cpd=cpd, max_buffer_bytes=int(max_buffer_bytes), n_gpu=1,
                             buffers_per_device=buffers_per_device, use_pinned=-1, path=PATHS[path],
-                            sinks_per_lane=sinks_per_lane, source_tile=0)
+                            sinks_per_lane=sinks_per_lane, source_tile=0, gpu_streams=gpu_streams,
+                            max_group=max_group)
         msb, mrb = C.c_int32(0), C.c_int32(0)
         check(lib().nf_setup(C.byref(p), C.byref(msb), C.byref(mrb)), "nf_setup")
         NearFieldDriver._active = self
@@ -281,6 +332,15 @@ class NearFieldDriver:
             stats.append(t.stats())
             t.release()
         return stats
+
+    def autotune_table(self):
+        """The autotuner's table: occupancy histogram -> chosen tile shape."""
+        import json
+        n = lib().nf_autotune_table(None, 0)
+        check(min(n, 0), "nf_autotune_table")
+        buf = C.create_string_buffer(n + 1)
+        lib().nf_autotune_table(buf, n + 1)
+        return json.loads(buf.value.decode())
 
     def close(self):
         if NearFieldDriver._active is self:

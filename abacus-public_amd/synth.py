@@ -123,20 +123,22 @@ def _grf_displacement(ng, rms, rng):
     dk = np.fft.rfftn(noise) * amp
     out = np.empty((ng ** 3, 3), np.float64)
     for a, ka in enumerate((kx, ky, kz)):
-        psi = np.fft.irfftn(1j * ka / k2 * dk, s=(ng, ng, ng))
+        psi = np.fft.irfftn(1j * ka / k2 * dk, s=(ng, ng, ng), axes=(0, 1, 2))
         out[:, a] = psi.ravel()
     out *= rms / np.sqrt((out ** 2).mean(axis=0).mean())
     return out
 
 
 def clustered_box(n1d, cpd, seed, halo_frac=0.3, n_halos=2000, rmin=0.02, rmax=0.3,
-                  mass_slope=1.9, rtrunc=10.0):
+                  mass_slope=1.9, mass_range=4.0, rtrunc=10.0):
     """The clustered recipe: N = n1d^3; 70% lattice + GRF displacement, 30% in
     Plummer spheres.  The lattice fraction is a uniform random 70% subset of
     the n1d^3 lattice.  Halo masses are drawn from dN/dM ~ M^-1.9 on
-    [1, 1e4] particle units and normalised to the halo particle budget; scale
-    radii are log-uniform in [rmin, rmax] cell widths, truncated at
-    ``rtrunc`` scale radii."""
+    [M0, mass_range * M0] and normalised to the halo particle budget, so the
+    mean halo holds halo_frac*N/n_halos particles (2,517 for 256^3) and the
+    largest about 2.8x that: cell occupancies then run from 0 to ~5,000.
+    Scale radii are log-uniform in [rmin, rmax] cell widths; each sphere is
+    truncated at ``rtrunc`` scale radii."""
     rng = np.random.default_rng(seed)
     N = n1d ** 3
     n_halo_parts = int(round(halo_frac * N))
@@ -149,7 +151,7 @@ def clustered_box(n1d, cpd, seed, halo_frac=0.3, n_halos=2000, rmin=0.02, rmax=0
     lat = (idx + 0.5) / n1d - 0.5 + disp[keep]
     # Plummer spheres
     a = 1.0 - mass_slope
-    lo, hi = 1.0, 1.0e4
+    lo, hi = 1.0, float(mass_range)
     u = rng.random(n_halos)
     m = (lo ** a + u * (hi ** a - lo ** a)) ** (1.0 / a)
     nh = np.floor(m / m.sum() * n_halo_parts).astype(np.int64)

@@ -135,6 +135,9 @@ typedef struct nf_task_stats {
     uint64_t direct_total, padded_direct_total;
     int32_t assigned_buffer;
     int32_t path;
+    int32_t group_size;            /* SICs that shared this task's persistent launch */
+    double group_direct_kernel_s;  /* GPU time of that shared direct-kernel launch */
+    uint64_t group_direct_total;   /* interactions of the whole group */
 } nf_task_stats;
 
 typedef struct nf_task nf_task;
@@ -149,6 +152,12 @@ typedef struct nf_setup_params {
     int32_t path;                 /* nf_path used by nf_submit */
     int32_t sinks_per_lane;       /* batched path register tile: 1, 2 or 4 (0: auto) */
     int32_t source_tile;          /* batched path TwoSum tile depth (0: auto) */
+    int32_t gpu_streams;          /* HIP streams the workers enqueue on: 0 one per
+                                     worker (copies overlap kernels); 1 one shared
+                                     stream (tasks run back to back on the GPU while
+                                     the workers still prepare them in parallel) */
+    int32_t max_group;            /* batched path: SICs queued together may share one
+                                     persistent launch, up to this many (0: 8) */
 } nf_setup_params;
 
 /* GetNGPU (GPUControl.cu:267-271). */
@@ -167,9 +176,10 @@ int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int32_t *max_so
 int nf_reset(void);
 
 /* SetInteractionCollection::GPUExecute (GPUControl.cu:235-261): queue the SIC
- * on the worker chosen by its y-midpoint (or run it on the caller's thread when
- * blocking != 0) and return a task handle.  The slabs and the plan arrays must
- * stay alive until the task completes. */
+ * for the workers and return a task handle; with blocking != 0, also wait for
+ * it.  On the batched path a worker takes every compatible SIC waiting in the
+ * queue (up to max_group) into one persistent launch.  The slabs and the plan
+ * arrays must stay alive until the task completes. */
 int nf_submit(const nf_sic *sic, int32_t blocking, nf_task **task);
 
 /* SetInteractionCollection::CheckCompletion (PencilOnPencil.cc:434-436):
@@ -187,6 +197,25 @@ int nf_release(nf_task *task);
 
 /* Thread-local description of the last error. */
 const char *nf_last_error(void);
+
+/* The batched-path autotuner's table (no counterpart in the reference).  With
+ * sinks_per_lane = 0 the runtime keys every launch group on the histogram of
+ * its sink-pencil occupancies; the first group with a new histogram times each
+ * candidate tile shape on its own launch and the fastest is kept for every
+ * later group with that histogram.  Writes the table as JSON into buf;
+ * returns the length it needs (excluding the terminator), or < 0. */
+int nf_autotune_table(char *buf, size_t len);
+
+/* ---- device memory for HBM-resident slabs ------------------------------
+ * No counterpart in the reference, whose slabs live in host memory.  These
+ * are for callers that keep PosXYZ and acceleration slabs resident on the GPU
+ * and submit SICs with mem = NF_MEM_DEVICE; they use the module's own HIP
+ * runtime so a caller needs no second GPU library. */
+int nf_device_alloc(uint64_t bytes, void **ptr);
+int nf_device_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device (synchronous). */
+int nf_device_copy(void *dst, const void *src, uint64_t bytes, int32_t kind);
+int nf_device_synchronize(void);
 
 /* ---- planning (host) ------------------------------------------------- */
 

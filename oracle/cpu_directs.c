@@ -139,3 +139,17 @@ uint64_t or_execute_slab_cpu(int cpd, const float *const *pos, const int64_t *ns
     return di;
 }
 
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads the OpenMP loops of this library use. */
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -13,16 +13,3 @@ for sub in ("abacus-public_amd", "oracle", ""):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
 
-
-def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
-
-
-@pytest.fixture(scope="session")
-def need_gpu():
-    if not gpu_available():
-        pytest.skip("no GPU")

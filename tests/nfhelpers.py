@@ -28,11 +28,15 @@ def rel_err(a, ref):
     return num / np.where(den > 0, den, 1.0)
 
 
-def summarize(rel, mask=None):
+def summarize(rel, mask=None, tail=False):
     if mask is not None:
         rel = rel[mask]
-    return dict(rms=float(np.sqrt(np.mean(rel ** 2))) if rel.size else 0.0,
-                max=float(rel.max()) if rel.size else 0.0, n=int(rel.size))
+    out = dict(rms=float(np.sqrt(np.mean(rel ** 2))) if rel.size else 0.0,
+               max=float(rel.max()) if rel.size else 0.0, n=int(rel.size))
+    if tail and rel.size:
+        out.update(p99=float(np.percentile(rel, 99)), p999=float(np.percentile(rel, 99.9)),
+                   above_1e5=int((rel > 1e-5).sum()))
+    return out
 
 
 def not_soft_dominated(ref, soft):

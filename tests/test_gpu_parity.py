@@ -145,20 +145,61 @@ def test_minimum_cpd_wraps():
 
 
 def test_device_resident_matches_host():
-    """NF_MEM_DEVICE (slabs and acceleration slab in HBM, torch-allocated)
-    gives bitwise the host-mode result."""
-    import torch
+    """NF_MEM_DEVICE (slabs and acceleration slab resident in HBM) gives
+    bitwise the host-mode result."""
     box = small_uniform(cpd=7, ppc=20, seed=21)
     eps, b2, _ = params(box)
     host = run_box(box, "batched", slabs=[3])[3]
-    dev = [torch.from_numpy(box.slab(x).pos).cuda() for x in range(box.cpd)]
-    acc = torch.zeros((box.slab(3).n, 4), dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
     with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
                              max_buffer_bytes=1 << 30) as drv:
-        ptrs = [dev[(3 - R + c) % box.cpd].data_ptr() for c in range(2 * R + 1)]
-        sic = drv.plan(box, 3, pos_ptrs=ptrs, acc_ptr=acc.data_ptr(), mem=nfm.NF_MEM_DEVICE)
+        dev = [nfm.DeviceArray(host=box.slab(x).pos) for x in range(box.cpd)]
+        acc = nfm.DeviceArray(nbytes=box.slab(3).n * 16)
+        acc.zero()
+        ptrs = [dev[(3 - R + c) % box.cpd].ptr for c in range(2 * R + 1)]
+        sic = drv.plan(box, 3, pos_ptrs=ptrs, acc_ptr=acc.ptr, mem=nfm.NF_MEM_DEVICE)
         t = drv.submit(sic)
         t.wait()
         t.release()
-    assert np.array_equal(acc.cpu().numpy(), host)
+        got = acc.to_host(np.float32, (box.slab(3).n, 4))
+        for d in dev + [acc]:
+            d.free()
+    assert np.array_equal(got, host)
+
+
+def test_launch_groups_match_single_launches(box9):
+    """Several SICs sharing one persistent launch give bitwise the results of
+    one launch per SIC (work items are independent; order does not matter)."""
+    eps, b2, _ = params(box9)
+    out = {}
+    for mg in (1, 8):
+        with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                                 max_buffer_bytes=2 << 30, sinks_per_lane=2, max_group=mg) as drv:
+            accs = {x: np.full((box9.slab(x).n, 4), np.nan, np.float32) for x in range(box9.cpd)}
+            for x in range(box9.cpd):
+                drv.execute_slab(box9, x, accs[x])
+            sizes = []
+            for x in range(box9.cpd):
+                sizes += [s["group_size"] for s in drv.finalize(x)]
+            out[mg] = (accs, sizes)
+    assert max(out[1][1]) == 1 and max(out[8][1]) > 1
+    for x in range(box9.cpd):
+        assert np.array_equal(out[1][0][x], out[8][0][x]), x
+
+
+def test_autotuner_keys_on_occupancy(box9, ref9):
+    eps, b2, _ = params(box9)
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                             max_buffer_bytes=2 << 30, sinks_per_lane=0, max_group=1) as drv:
+        accs = {}
+        for x in (0, 4, 8, 2):
+            accs[x] = np.full((box9.slab(x).n, 4), np.nan, np.float32)
+            drv.execute_slab(box9, x, accs[x], blocking=True)
+            drv.finalize(x)
+        table = drv.autotune_table()
+    assert table and all(e["sinks_per_lane"] in (1, 2, 4) for e in table)
+    assert sum(e["uses"] for e in table) == 4
+    assert all(len(e["trial_ms"]) == len(e["candidates"]) for e in table)
+    for x in (0, 4, 8):
+        ref, soft = ref9[x]
+        st = summarize(rel_err(accs[x], ref), not_soft_dominated(ref, soft))
+        assert st["rms"] < 1e-6 and st["max"] < 1e-5, (x, st)
