@@ -59,37 +59,39 @@ __device__ __forceinline__ int cell_of(int i, const int *pre, int W) {
     return c;
 }
 
+// One wave per padded 64-sink block (the reference's SinkBlockParentPencil
+// names its pencil), so a heavy pencil spreads over many waves.
 template <bool F64>
 __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *__restrict__ slab, uint64_t n,
                                                       float4 *__restrict__ out, float cellsize, double invcpd) {
-    const int s = blockIdx.x;
+    const int b = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+    if (b >= pl.nsink_blocks) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int s = pl.parent[b];
     const int W = pl.W;
     const nf_sink_plan p = pl.sink_plan[s];
     int pre[NF_MAX_WIDTH + 1];
     pre[0] = 0;
     for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
     const int count = pre[W];
-    const int padded = (count + kWave - 1) / kWave * kWave;
-    const int64_t o = pl.sink_set_start[s];
-    for (int i = threadIdx.x; i < padded; i += blockDim.x) {
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < count) {
-            const int c = cell_of(i, pre, W);
-            const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
-            v.x = slab[idx];
-            v.y = slab[n + idx];
-            float z = slab[2 * n + idx];
-            const int dc = c - pl.R;
-            if (F64) {
-                z = (float)((double)z + (double)dc * invcpd);
-            } else {  // PencilPlan.cc:43-50: dz = (c-R)*cellsize in FLOAT, added if nonzero
-                const float dz = (float)dc * cellsize;
-                if (dz != 0.f) z = z + dz;
-            }
-            v.z = z;
+    const int i = b * kWave - pl.sink_set_start[s] + lane;  // index within the pencil
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < count) {
+        const int c = cell_of(i, pre, W);
+        const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
+        v.x = slab[idx];
+        v.y = slab[n + idx];
+        float z = slab[2 * n + idx];
+        const int dc = c - pl.R;
+        if (F64) {
+            z = (float)((double)z + (double)dc * invcpd);
+        } else {  // PencilPlan.cc:43-50: dz = (c-R)*cellsize in FLOAT, added if nonzero
+            const float dz = (float)dc * cellsize;
+            if (dz != 0.f) z = z + dz;
         }
-        out[o + i] = v;
+        v.z = z;
     }
+    out[(int64_t)b * kWave + lane] = v;
 }
 
 __device__ __forceinline__ const float *slab_ptr(const DevSlabs &sl, int c) {
@@ -108,50 +110,48 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// One workgroup per source pencil; each wave writes whole 64-source blocks and
-// their bounding boxes (padding excluded).  box[2b] = min, box[2b+1] = max.
+// One wave per padded 64-source block; it writes the block and its bounding
+// box (padding excluded): box[2b] = min, box[2b+1] = max.
 template <bool F64>
 __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
                                                         float4 *__restrict__ box, float cellsize,
                                                         double invcpd) {
-    const int s = blockIdx.x;
+    const int b = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+    if (b >= pl.nsrc_blocks) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int s = pl.src_parent[b];
+    if (s < 0) return;  // a block no pencil covers (never read)
     const int W = pl.W;
     const nf_source_plan p = pl.src_plan[s];
     int pre[NF_MAX_WIDTH + 1];
     pre[0] = 0;
     for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
     const int count = pre[W];
-    const int nblk = (count + kWave - 1) / kWave;
-    const int64_t o = pl.src_set_start[s];
-    const int lane = threadIdx.x & (kWave - 1);
-    for (int b = threadIdx.x / kWave; b < nblk; b += blockDim.x / kWave) {
-        const int i = b * kWave + lane;
-        float4 v = make_float4(kFar, 0.f, 0.f, 0.f);
-        const bool real = i < count;
-        if (real) {
-            const int c = cell_of(i, pre, W);
-            const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
-            const float *P = slab_ptr(sl, c);
-            const uint64_t m = slab_n(sl, c);
-            float x = P[idx];
-            const int dc = c - pl.R;
-            if (F64) {
-                x = (float)((double)x + (double)dc * invcpd);
-            } else {  // PencilPlan.cc:226-238
-                const float dx = (float)dc * cellsize;
-                if (dx != 0.f) x = x + dx;
-            }
-            v = make_float4(x, P[m + idx], P[2 * m + idx], 0.f);
+    const int i = b * kWave - pl.src_set_start[s] + lane;
+    float4 v = make_float4(kFar, 0.f, 0.f, 0.f);
+    const bool real = i < count;
+    if (real) {
+        const int c = cell_of(i, pre, W);
+        const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
+        const float *P = slab_ptr(sl, c);
+        const uint64_t m = slab_n(sl, c);
+        float x = P[idx];
+        const int dc = c - pl.R;
+        if (F64) {
+            x = (float)((double)x + (double)dc * invcpd);
+        } else {  // PencilPlan.cc:226-238
+            const float dx = (float)dc * cellsize;
+            if (dx != 0.f) x = x + dx;
         }
-        out[o + i] = v;
-        const float inf = __builtin_huge_valf();
-        const float mnx = wave_min(real ? v.x : inf), mny = wave_min(real ? v.y : inf), mnz = wave_min(real ? v.z : inf);
-        const float mxx = wave_max(real ? v.x : -inf), mxy = wave_max(real ? v.y : -inf), mxz = wave_max(real ? v.z : -inf);
-        if (lane == 0) {
-            const int64_t gb = o / kWave + b;
-            box[2 * gb] = make_float4(mnx, mny, mnz, 0.f);
-            box[2 * gb + 1] = make_float4(mxx, mxy, mxz, 0.f);
-        }
+        v = make_float4(x, P[m + idx], P[2 * m + idx], 0.f);
+    }
+    out[(int64_t)b * kWave + lane] = v;
+    const float inf = __builtin_huge_valf();
+    const float mnx = wave_min(real ? v.x : inf), mny = wave_min(real ? v.y : inf), mnz = wave_min(real ? v.z : inf);
+    const float mxx = wave_max(real ? v.x : -inf), mxy = wave_max(real ? v.y : -inf), mxz = wave_max(real ? v.z : -inf);
+    if (lane == 0) {
+        box[2 * (int64_t)b] = make_float4(mnx, mny, mnz, 0.f);
+        box[2 * (int64_t)b + 1] = make_float4(mxx, mxy, mxz, 0.f);
     }
 }
 
@@ -256,10 +256,15 @@ __device__ __forceinline__ void partial_slots(const DevPlan &pl, int j, int z, i
 
 // FP32 co-add in the reference's order: the pencils of a skewer are applied
 // in increasing k, the first one setting the value and the rest adding.  For
-// cell z that is c = min(z, W-1) .. 0, then W-1 .. z+1.
-__global__ __launch_bounds__(64) void k_coadd_plain(DevPlan pl, const float4 *__restrict__ part,
-                                                    float4 *__restrict__ acc) {
-    const int z = blockIdx.x, j = blockIdx.y;
+// cell z that is c = min(z, W-1) .. 0, then W-1 .. z+1.  One wave per chunk
+// of 64 particles of a sink cell.
+__global__ __launch_bounds__(256) void k_coadd_plain(DevPlan pl, const float4 *__restrict__ part,
+                                                     float4 *__restrict__ acc) {
+    const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+    if (q >= pl.n_coadd_chunks) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int cell = pl.coadd_chunk[2 * q], first = pl.coadd_chunk[2 * q + 1];
+    const int j = cell / pl.Nk, z = cell % pl.Nk;
     int64_t base[NF_MAX_WIDTH];
     uint32_t cstart;
     int cn;
@@ -268,41 +273,44 @@ __global__ __launch_bounds__(64) void k_coadd_plain(DevPlan pl, const float4 *__
     int order[NF_MAX_WIDTH], no = 0;
     for (int c = (z < W - 1 ? z : W - 1); c >= 0; c--) order[no++] = c;
     for (int c = W - 1; c > z; c--) order[no++] = c;
-    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
-        float4 a = part[base[order[0]] + i];
-        for (int t = 1; t < W; t++) {
-            const float4 p = part[base[order[t]] + i];
-            a.x += p.x;
-            a.y += p.y;
-            a.z += p.z;
-            a.w += p.w;
-        }
-        acc[cstart + i] = a;
+    const int i = first + lane;
+    if (i >= cn) return;
+    float4 a = part[base[order[0]] + i];
+    for (int t = 1; t < W; t++) {
+        const float4 p = part[base[order[t]] + i];
+        a.x += p.x;
+        a.y += p.y;
+        a.z += p.z;
+        a.w += p.w;
     }
+    acc[cstart + i] = a;
 }
 
 // FP64 co-add: sum of the W double partials, scaled back to the reference's
-// units (out_scale applies to a, not to w).
-__global__ __launch_bounds__(64) void k_coadd_f64(DevPlan pl, const double *__restrict__ part, double out_scale,
-                                                  float4 *__restrict__ acc) {
-    const int z = blockIdx.x, j = blockIdx.y;
+// units (out_scale applies to a, not to w).  One wave per 64-particle chunk.
+__global__ __launch_bounds__(256) void k_coadd_f64(DevPlan pl, const double *__restrict__ part, double out_scale,
+                                                   float4 *__restrict__ acc) {
+    const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+    if (q >= pl.n_coadd_chunks) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int cell = pl.coadd_chunk[2 * q], first = pl.coadd_chunk[2 * q + 1];
+    const int j = cell / pl.Nk, z = cell % pl.Nk;
     int64_t base[NF_MAX_WIDTH];
     uint32_t cstart;
     int cn;
     partial_slots(pl, j, z, base, &cstart, &cn);
     const int W = pl.W;
-    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        for (int c = 0; c < W; c++) {
-            const double *p = part + 4 * (base[c] + i);
-            a0 += p[0];
-            a1 += p[1];
-            a2 += p[2];
-            a3 += p[3];
-        }
-        acc[cstart + i] = make_float4((float)(a0 * out_scale), (float)(a1 * out_scale), (float)(a2 * out_scale),
-                                      (float)a3);
+    const int i = first + lane;
+    if (i >= cn) return;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int c = 0; c < W; c++) {
+        const double *p = part + 4 * (base[c] + i);
+        a0 += p[0];
+        a1 += p[1];
+        a2 += p[2];
+        a3 += p[3];
     }
+    acc[cstart + i] = make_float4((float)(a0 * out_scale), (float)(a1 * out_scale), (float)(a2 * out_scale), (float)a3);
 }
 
 // ---------------------------------------------------------------------------
@@ -669,20 +677,22 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void
     const float cellsize = (float)(1.0 / (double)pl.cpd);
     const double invcpd = 1.0 / (double)pl.cpd;
     float4 *box = static_cast<float4 *>(box_after(sources4, pl.nsrc_blocks));
-    if (pl.nsink_sets > 0) {
+    if (pl.nsink_blocks > 0) {
+        const dim3 grid((pl.nsink_blocks + 3) / 4);
         if (fp64_offsets)
-            hipLaunchKernelGGL(k_gather_sinks<true>, dim3(pl.nsink_sets), dim3(256), 0, st, pl, sl.pos[pl.R],
-                               sl.n[pl.R], static_cast<float4 *>(sinks4), cellsize, invcpd);
+            hipLaunchKernelGGL(k_gather_sinks<true>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R],
+                               static_cast<float4 *>(sinks4), cellsize, invcpd);
         else
-            hipLaunchKernelGGL(k_gather_sinks<false>, dim3(pl.nsink_sets), dim3(256), 0, st, pl, sl.pos[pl.R],
-                               sl.n[pl.R], static_cast<float4 *>(sinks4), cellsize, invcpd);
+            hipLaunchKernelGGL(k_gather_sinks<false>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R],
+                               static_cast<float4 *>(sinks4), cellsize, invcpd);
     }
-    if (pl.nsrc_sets > 0) {
+    if (pl.nsrc_blocks > 0) {
+        const dim3 grid((pl.nsrc_blocks + 3) / 4);
         if (fp64_offsets)
-            hipLaunchKernelGGL(k_gather_sources<true>, dim3(pl.nsrc_sets), dim3(256), 0, st, pl, sl,
-                               static_cast<float4 *>(sources4), box, cellsize, invcpd);
+            hipLaunchKernelGGL(k_gather_sources<true>, grid, dim3(256), 0, st, pl, sl, static_cast<float4 *>(sources4),
+                               box, cellsize, invcpd);
         else
-            hipLaunchKernelGGL(k_gather_sources<false>, dim3(pl.nsrc_sets), dim3(256), 0, st, pl, sl,
+            hipLaunchKernelGGL(k_gather_sources<false>, grid, dim3(256), 0, st, pl, sl,
                                static_cast<float4 *>(sources4), box, cellsize, invcpd);
     }
     return (int)hipGetLastError();
@@ -704,17 +714,17 @@ int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const vo
 }
 
 int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, void *stream) {
-    if (pl.j_width == 0) return 0;
+    if (pl.n_coadd_chunks == 0) return 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_coadd_plain, dim3(pl.cpd, pl.j_width), dim3(kWave), 0, st, pl,
+    hipLaunchKernelGGL(k_coadd_plain, dim3((pl.n_coadd_chunks + 3) / 4), dim3(256), 0, st, pl,
                        static_cast<const float4 *>(partial4), reinterpret_cast<float4 *>(acc4));
     return (int)hipGetLastError();
 }
 
 int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4, void *stream) {
-    if (pl.j_width == 0) return 0;
+    if (pl.n_coadd_chunks == 0) return 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_coadd_f64, dim3(pl.cpd, pl.j_width), dim3(kWave), 0, st, pl,
+    hipLaunchKernelGGL(k_coadd_f64, dim3((pl.n_coadd_chunks + 3) / 4), dim3(256), 0, st, pl,
                        static_cast<const double *>(partial_d4), out_scale, reinterpret_cast<float4 *>(acc4));
     return (int)hipGetLastError();
 }

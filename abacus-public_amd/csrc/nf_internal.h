@@ -53,8 +53,11 @@ struct DevPlan {
     const int32_t *parent;
     const int32_t *ilist;
     const float *yoff;
+    const int32_t *src_parent;   // source pencil of each padded 64-source block
+    const int32_t *coadd_chunk;  // (j*Nk + z, first particle) per 64-particle chunk of a sink cell
     int32_t nsink_sets, nsrc_sets, nsink_blocks, nsrc_blocks;
     int32_t W, R, cpd, Nk, j_width, j_low;
+    int32_t n_coadd_chunks;
 };
 
 // The slabs of one SIC on the device.

@@ -145,6 +145,8 @@ struct Worker {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec = nullptr;  // the last group's executable graph (use_graphs)
+    uint64_t graphs_instantiated = 0, graphs_updated = 0;
     DevBuf plan, work, slabs;
     PinBuf pin_plan, pin_acc;
     std::thread th;
@@ -173,6 +175,30 @@ struct Runtime {
     std::map<std::string, TuneEntry> tune;  // occupancy-histogram key -> tuned shape
     hipStream_t shared = nullptr;  // when gpu_streams == 1: the one stream all workers use
     std::mutex enqueue;            // orders whole task sequences on the shared stream
+
+    // Host-mode slabs resident in HBM.  A slot is reusable by address only
+    // inside the stream session it was uploaded in (nf_stream_begin/end).
+    struct Slot {
+        const float *host = nullptr;
+        uint64_t n = 0;
+        uint64_t epoch = 0;
+        bool valid = false;
+        int refs = 0;
+        uint64_t last = 0;
+        DevBuf buf;
+        hipEvent_t ready = nullptr;
+        hipEvent_t t0 = nullptr, t1 = nullptr;  // brackets the last upload
+    };
+    struct SlabCache {
+        std::mutex m;
+        bool active = false;
+        uint64_t epoch = 0, clock = 0;
+        std::vector<std::unique_ptr<Slot>> slots;
+        hipStream_t up = nullptr, down = nullptr;  // uploads; acceleration copy-backs
+        PinBuf stage[2];                           // double-buffered pinned staging
+        hipEvent_t stage_done[2] = {nullptr, nullptr};
+        int next = 0;
+    } cache;
 };
 
 std::mutex g_rt_mutex;
@@ -183,8 +209,17 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // Where one SIC's plan arrays sit inside a group's plan upload.
 struct PlanLayout {
     size_t o_sink_start, o_sink_count, o_idmax, o_sink_plan, o_src_start, o_src_count, o_src_plan, o_parent,
-        o_ilist, o_yoff, total;
+        o_ilist, o_yoff, o_src_parent, o_chunks, total;
+    int32_t n_chunks;
 };
+
+// 64-particle chunks of the sink cells of rows [j_low, j_high): the co-add's
+// work list.  Pencil k = z starts with cell z, so its first cell is cell z.
+int32_t count_coadd_chunks(const nf_sic &s) {
+    int64_t n = 0;
+    for (int k = 0; k < s.NSinkSets; k++) n += (s.SinkPlan[k].cell[0].N + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE;
+    return (int32_t)n;
+}
 
 PlanLayout plan_layout(const nf_sic &s, size_t base) {
     PlanLayout L{};
@@ -204,6 +239,9 @@ PlanLayout plan_layout(const nf_sic &s, size_t base) {
     L.o_parent = take(4 * (size_t)s.NSinkBlocks);
     L.o_ilist = take(4 * (size_t)s.InteractionCount);
     L.o_yoff = take(4 * (size_t)s.InteractionCount);
+    L.o_src_parent = take(4 * (size_t)s.NSourceBlocks);
+    L.n_chunks = count_coadd_chunks(s);
+    L.o_chunks = take(8 * (size_t)L.n_chunks);
     L.total = o - base;
     return L;
 }
@@ -319,9 +357,16 @@ int validate(const nf_sic &s) {
     for (int i = 0; i < s.InteractionCount; i++)
         if (s.SinkSourceInteractionList[i] < 0 || s.SinkSourceInteractionList[i] >= s.NSourceSets)
             return fail(NF_ERR_INVALID, "interaction list entry out of range");
-    for (int b = 0; b < s.NSinkBlocks; b++)
-        if (s.SinkBlockParentPencil[b] < 0 || s.SinkBlockParentPencil[b] >= s.NSinkSets)
-            return fail(NF_ERR_INVALID, "SinkBlockParentPencil out of range");
+    // Each padded block must lie inside the padded range of the pencil it
+    // names: the sink gather indexes the pencil by the block's offset.
+    for (int b = 0; b < s.NSinkBlocks; b++) {
+        const int k = s.SinkBlockParentPencil[b];
+        if (k < 0 || k >= s.NSinkSets) return fail(NF_ERR_INVALID, "SinkBlockParentPencil out of range");
+        const int64_t lo = s.SinkSetStart[k];
+        const int64_t hi = lo + (int64_t)(s.SinkSetCount[k] + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE * NF_BLOCK_SIZE;
+        if ((int64_t)b * NF_BLOCK_SIZE < lo || (int64_t)b * NF_BLOCK_SIZE >= hi)
+            return fail(NF_ERR_INVALID, "SinkBlockParentPencil names a pencil that does not hold the block");
+    }
     // The co-add reads pencil k = z's first cell as cell z of the skewer:
     // the plans must follow the reference's pencil geometry.
     for (int j = 0; j < s.j_width; j++)
@@ -368,6 +413,80 @@ std::string occupancy_key(const std::vector<Task *> &g) {
     std::ostringstream os;
     for (int b = 0; b < kBins; b++) os << (b ? "," : "") << (int)std::lround(16.0 * h[b] / std::max(tot, 1.0));
     return os.str();
+}
+
+// ---- Host-slab cache -------------------------------------------------
+
+// Find the HBM copy of a host slab (inside a stream session) or upload it
+// into the least recently used free slot.  Returns the slot with a reference
+// held; *uploaded says whether it was copied now (its t0/t1 events then
+// bracket the copy).  The upload is enqueued on the cache's upload stream
+// through pinned staging while the cache lock is held, so a second user of
+// the same slab finds its ready event recorded and waits on it on the GPU.
+int acquire_slab(Runtime &rt, const float *host, uint64_t n, Runtime::Slot **out, bool *uploaded) {
+    auto &C = rt.cache;
+    std::lock_guard<std::mutex> lk(C.m);
+    *uploaded = false;
+    if (C.active)
+        for (auto &sp : C.slots) {
+            Runtime::Slot &sl = *sp;
+            if (sl.valid && sl.host == host && sl.n == n && sl.epoch == C.epoch) {
+                sl.refs++;
+                sl.last = ++C.clock;
+                *out = &sl;
+                return NF_OK;
+            }
+        }
+    Runtime::Slot *v = nullptr;
+    for (auto &sp : C.slots)
+        if (sp->refs == 0 && (!v || sp->last < v->last)) v = sp.get();
+    if (!v) {
+        C.slots.push_back(std::make_unique<Runtime::Slot>());
+        v = C.slots.back().get();
+        NF_HIP(hipEventCreateWithFlags(&v->ready, hipEventDisableTiming));
+        NF_HIP(hipEventCreate(&v->t0));
+        NF_HIP(hipEventCreate(&v->t1));
+    }
+    v->valid = false;
+    v->refs = 1;
+    v->last = ++C.clock;
+    v->host = host;
+    v->n = n;
+    v->epoch = C.epoch;
+    const size_t bytes = 12 * (size_t)n;
+    int rc = v->buf.ensure(bytes);
+    if (rc) {
+        v->refs = 0;
+        return rc;
+    }
+    if (bytes) {
+        const int k = C.next;
+        C.next ^= 1;
+        NF_HIP(hipEventSynchronize(C.stage_done[k]));  // the previous copy out of this buffer is done
+        if ((rc = C.stage[k].ensure(bytes))) {
+            v->refs = 0;
+            return rc;
+        }
+        std::memcpy(C.stage[k].p, host, bytes);
+        NF_HIP(hipEventRecord(v->t0, C.up));
+        NF_HIP(hipMemcpyAsync(v->buf.p, C.stage[k].p, bytes, hipMemcpyHostToDevice, C.up));
+        NF_HIP(hipEventRecord(v->t1, C.up));
+        NF_HIP(hipEventRecord(C.stage_done[k], C.up));
+    } else {
+        NF_HIP(hipEventRecord(v->t0, C.up));
+        NF_HIP(hipEventRecord(v->t1, C.up));
+    }
+    NF_HIP(hipEventRecord(v->ready, C.up));
+    v->valid = true;
+    *uploaded = true;
+    *out = v;
+    return NF_OK;
+}
+
+void release_slab(Runtime &rt, Runtime::Slot *sl) {
+    auto &C = rt.cache;
+    std::lock_guard<std::mutex> lk(C.m);
+    if (--sl->refs == 0 && !(C.active && sl->epoch == C.epoch)) sl->valid = false;
 }
 
 // Execute a group of compatible SICs on worker w: one plan upload, a gather
@@ -455,10 +574,10 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         WL[i] = work_layout(g[i]->sic, path, work_total);
         work_total += WL[i].total;
     }
-    // Host mode: every distinct host slab once, plus an acceleration slab per SIC.
+    // Host mode: every distinct host slab of the group once (through the
+    // slab cache), plus a device acceleration slab per SIC.
     std::vector<const float *> hslab;
     std::vector<uint64_t> hslab_n;
-    std::vector<size_t> hslab_off;
     std::vector<size_t> acc_off(n, 0);
     size_t slab_total = 0;
     if (s0.mem == NF_MEM_HOST) {
@@ -468,8 +587,6 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
                 if (std::find(hslab.begin(), hslab.end(), s.SourcePosSlab[c]) != hslab.end()) continue;
                 hslab.push_back(s.SourcePosSlab[c]);
                 hslab_n.push_back(s.Nslab[c]);
-                hslab_off.push_back(slab_total);
-                slab_total = align_up(slab_total + 12 * (size_t)s.Nslab[c], 256);
             }
             acc_off[i] = slab_total;
             slab_total = align_up(slab_total + 16 * (size_t)s.Nslab[s.nfradius], 256);
@@ -481,6 +598,26 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) || (rc = w.pin_plan.ensure(plan_total)))
         return rc;
     if (s0.mem == NF_MEM_HOST && (rc = w.slabs.ensure(slab_total))) return rc;
+
+    // Slabs: HBM copies through the cache (uploads overlap other groups' kernels).
+    std::vector<Runtime::Slot *> hslot(hslab.size(), nullptr);
+    struct SlotRelease {  // references are dropped when the group is done
+        Runtime &rt;
+        std::vector<Runtime::Slot *> &v;
+        ~SlotRelease() {
+            for (auto *x : v)
+                if (x) release_slab(rt, x);
+        }
+    } slot_release{rt, hslot};
+    int n_up = 0, n_reuse = 0;
+    std::vector<char> fresh(hslab.size(), 0);
+    for (size_t k = 0; k < hslab.size(); k++) {
+        bool up = false;
+        if ((rc = acquire_slab(rt, hslab[k], hslab_n[k], &hslot[k], &up))) return rc;
+        fresh[k] = up;
+        if (up) n_up++;
+        else n_reuse++;
+    }
 
     // ---- Fill the pinned plan image.
     char *hp = static_cast<char *>(w.pin_plan.p);
@@ -504,6 +641,23 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         if (s.NSinkBlocks) std::memcpy(hp + L.o_parent, s.SinkBlockParentPencil, 4 * (size_t)s.NSinkBlocks);
         std::memcpy(hp + L.o_ilist, s.SinkSourceInteractionList, 4 * (size_t)s.InteractionCount);
         std::memcpy(hp + L.o_yoff, s.SinkSourceYOffset, 4 * (size_t)s.InteractionCount);
+        {   // block -> source pencil, and the co-add chunks
+            int32_t *sp = reinterpret_cast<int32_t *>(hp + L.o_src_parent);
+            std::fill(sp, sp + s.NSourceBlocks, -1);  // blocks of no pencil are skipped
+            for (int k = 0; k < s.NSourceSets; k++) {
+                const int b0 = s.SourceSetStart[k] / NF_BLOCK_SIZE;
+                const int nb = (s.SourceSetCount[k] + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE;
+                for (int b = 0; b < nb; b++) sp[b0 + b] = k;
+            }
+            int32_t *ch = reinterpret_cast<int32_t *>(hp + L.o_chunks);
+            int64_t q = 0;
+            for (int k = 0; k < s.NSinkSets; k++)
+                for (int f = 0; f < s.SinkPlan[k].cell[0].N; f += NF_BLOCK_SIZE) {
+                    ch[2 * q] = k;
+                    ch[2 * q + 1] = f;
+                    q++;
+                }
+        }
 
         DevPlan &pl = plans[i];
         pl.sink_set_start = reinterpret_cast<const int32_t *>(pd + L.o_sink_start);
@@ -516,6 +670,9 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         pl.parent = reinterpret_cast<const int32_t *>(pd + L.o_parent);
         pl.ilist = reinterpret_cast<const int32_t *>(pd + L.o_ilist);
         pl.yoff = reinterpret_cast<const float *>(pd + L.o_yoff);
+        pl.src_parent = reinterpret_cast<const int32_t *>(pd + L.o_src_parent);
+        pl.coadd_chunk = reinterpret_cast<const int32_t *>(pd + L.o_chunks);
+        pl.n_coadd_chunks = L.n_chunks;
         pl.nsink_sets = s.NSinkSets;
         pl.nsrc_sets = s.NSourceSets;
         pl.nsink_blocks = s.NSinkBlocks;
@@ -532,7 +689,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             sl.n[c] = s.Nslab[c];
             if (s.mem == NF_MEM_HOST) {
                 const size_t k = std::find(hslab.begin(), hslab.end(), s.SourcePosSlab[c]) - hslab.begin();
-                sl.pos[c] = reinterpret_cast<const float *>(db + hslab_off[k]);
+                sl.pos[c] = static_cast<const float *>(hslot[k]->buf.p);
             } else {
                 sl.pos[c] = s.SourcePosSlab[c];
             }
@@ -563,18 +720,35 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     std::unique_lock<std::mutex> enqueue_lock(rt.enqueue, std::defer_lock);
     if (rt.shared) enqueue_lock.lock();
     uint64_t to_dev = plan_total, from_dev = 0;
+    // With graphs, the sequence below is captured and launched as one graph.
+    const bool use_graph = rt.params.use_graphs && !trial;
+    struct CaptureGuard {  // ends an open capture on every early return
+        hipStream_t s;
+        bool on = false;
+        ~CaptureGuard() {
+            if (on) {
+                hipGraph_t gr = nullptr;
+                if (hipStreamEndCapture(s, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
+            }
+        }
+    } capture{stream};
     NF_HIP(hipEventRecord(w.ev[0], stream));
     NF_HIP(hipMemcpyAsync(w.plan.p, hp, plan_total, hipMemcpyHostToDevice, stream));
-    for (size_t k = 0; k < hslab.size(); k++) {
-        NF_HIP(hipMemcpyAsync(db + hslab_off[k], hslab[k], 12 * (size_t)hslab_n[k], hipMemcpyHostToDevice, stream));
-        to_dev += 12 * hslab_n[k];
-    }
+    for (size_t k = 0; k < hslot.size(); k++) NF_HIP(hipStreamWaitEvent(stream, hslot[k]->ready, 0));
+    for (size_t k = 0; k < hslab.size(); k++) to_dev += 12 * hslab_n[k];
     NF_HIP(hipEventRecord(w.ev[1], stream));
+    // The captured region holds kernels only; copies and timing events stay
+    // outside it, around the graph launch.
+    if (use_graph) {
+        (void)hipGetLastError();
+        NF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        capture.on = true;
+    }
     int e = 0;
     for (int i = 0; i < n && !e; i++)
         e = launch_gather(plans[i], slabs[i], path == NF_PATH_BATCHED, wb + WL[i].o_sinks, wb + WL[i].o_src, stream);
     if (e) return fail(NF_ERR_HIP, std::string("gather launch: ") + hipGetErrorString((hipError_t)e));
-    NF_HIP(hipEventRecord(w.ev[2], stream));
+    if (!use_graph) NF_HIP(hipEventRecord(w.ev[2], stream));
     std::vector<hipEvent_t> tev;
     if (path == NF_PATH_BATCHED) {
         int32_t *counter = reinterpret_cast<int32_t *>(pd + o_counter);
@@ -596,7 +770,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             }
         }
         if (!e) {
-            NF_HIP(hipEventRecord(w.ev[2], stream));
+            if (trial) NF_HIP(hipEventRecord(w.ev[2], stream));
             e = launch(ts);
         }
     } else {
@@ -605,17 +779,47 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
                              stream);
     }
     if (e) return fail(NF_ERR_HIP, std::string("direct kernel launch: ") + hipGetErrorString((hipError_t)e));
-    NF_HIP(hipEventRecord(w.ev[3], stream));
+    if (!use_graph) NF_HIP(hipEventRecord(w.ev[3], stream));
     for (int i = 0; i < n && !e; i++) {
         if (path == NF_PATH_BATCHED) e = launch_coadd_f64(plans[i], wb + WL[i].o_part, pc.out_scale, accd[i], stream);
         else e = launch_coadd_plain(plans[i], wb + WL[i].o_part, accd[i], stream);
     }
     if (e) return fail(NF_ERR_HIP, std::string("co-add launch: ") + hipGetErrorString((hipError_t)e));
+    if (use_graph) {
+        hipGraph_t graph = nullptr;
+        capture.on = false;
+        NF_HIP(hipStreamEndCapture(stream, &graph));
+        bool updated = false;
+        if (w.gexec) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(w.gexec, graph, &err_node, &res) == hipSuccess &&
+                      res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGraphExecDestroy(w.gexec);
+                w.gexec = nullptr;
+            }
+        }
+        hipError_t ge = hipSuccess;
+        if (!updated) ge = hipGraphInstantiate(&w.gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ge != hipSuccess) {
+            w.gexec = nullptr;
+            return fail(NF_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ge));
+        }
+        if (updated) w.graphs_updated++;
+        else w.graphs_instantiated++;
+        NF_HIP(hipGraphLaunch(w.gexec, stream));
+    }
     NF_HIP(hipEventRecord(w.ev[4], stream));
     if (enqueue_lock.owns_lock()) enqueue_lock.unlock();
     NF_HIP(hipEventSynchronize(w.ev[4]));
     const double kern_ms = event_ms(w.ev[1], w.ev[4]);
-    const double direct_ms = event_ms(w.ev[2], w.ev[3]);
+    // A graph is timed as a whole (gathers and co-adds included).
+    const double direct_ms = use_graph ? kern_ms : event_ms(w.ev[2], w.ev[3]);
+    double upload_ms = 0.0;  // the slabs this group copied (they finished before its kernels)
+    for (size_t k = 0; k < hslot.size(); k++)
+        if (fresh[k]) upload_ms += event_ms(hslot[k]->t0, hslot[k]->t1);
     if (trial) {
         Runtime::TuneEntry te;
         int best = 0;
@@ -632,7 +836,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     }
 
     // ---- Host mode: the SICs' sink rows back into their AccSlabs.
-    double copyback = 0.0;
+    double copyback = 0.0, download_ms = 0.0;
     if (s0.mem == NF_MEM_HOST) {
         const auto tc = clk::now();
         for (int i = 0; i < n; i++) {
@@ -645,11 +849,14 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             const uint64_t r1 = (uint64_t)last.cell[0].start + (uint64_t)last.cell[0].N;
             if (r1 <= r0) continue;
             if ((rc = w.pin_acc.ensure(16 * (r1 - r0)))) return rc;
-            if (rt.shared) enqueue_lock.lock();
-            NF_HIP(hipMemcpyAsync(w.pin_acc.p, accd[i] + 4 * r0, 16 * (r1 - r0), hipMemcpyDeviceToHost, stream));
-            NF_HIP(hipEventRecord(w.ev[0], stream));
-            if (enqueue_lock.owns_lock()) enqueue_lock.unlock();
-            NF_HIP(hipEventSynchronize(w.ev[0]));
+            // The copy-back runs on the download stream, after the group's
+            // kernels, while the next group's kernels use the compute stream.
+            NF_HIP(hipEventRecord(w.ev[0], rt.cache.down));
+            NF_HIP(hipMemcpyAsync(w.pin_acc.p, accd[i] + 4 * r0, 16 * (r1 - r0), hipMemcpyDeviceToHost,
+                                  rt.cache.down));
+            NF_HIP(hipEventRecord(w.ev[2], rt.cache.down));
+            NF_HIP(hipEventSynchronize(w.ev[2]));
+            download_ms += event_ms(w.ev[0], w.ev[2]);
             std::memcpy(s.SinkAccSlab + 4 * r0, w.pin_acc.p, 16 * (r1 - r0));
             from_dev += 16 * (r1 - r0);
         }
@@ -676,6 +883,11 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         st.group_size = n;
         st.group_direct_kernel_s = direct_ms * 1e-3;
         st.group_direct_total = gdi;
+        st.graph = use_graph ? 1 : 0;
+        st.upload_s = upload_ms * 1e-3 * share;
+        st.download_s = download_ms * 1e-3 * share;
+        st.slabs_uploaded = n_up;
+        st.slabs_reused = n_reuse;
     }
     return NF_OK;
 }
@@ -743,6 +955,8 @@ void destroy_worker(Worker &w) {
     w.slabs.release();
     w.pin_plan.release();
     w.pin_acc.release();
+    if (w.gexec) (void)hipGraphExecDestroy(w.gexec);
+    w.gexec = nullptr;
     for (auto &e : w.ev)
         if (e) (void)hipEventDestroy(e);
     if (w.stream) (void)hipStreamDestroy(w.stream);
@@ -831,6 +1045,16 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
         }
         rt->workers.push_back(std::move(w));
     }
+    {
+        auto &C = rt->cache;
+        hipError_t e = hipStreamCreateWithFlags(&C.up, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&C.down, hipStreamNonBlocking);
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&C.stage_done[k], hipEventDisableTiming);
+        if (e != hipSuccess) {
+            for (auto &x : rt->workers) destroy_worker(*x);
+            return fail(NF_ERR_HIP, std::string("copy stream setup: ") + hipGetErrorString(e));
+        }
+    }
     if (p->gpu_streams == 1) {
         hipError_t e = hipStreamCreateWithFlags(&rt->shared, hipStreamNonBlocking);
         if (e != hipSuccess) {
@@ -860,6 +1084,23 @@ extern "C" int nf_reset(void) {
     if (g_rt->shared) {
         (void)hipStreamSynchronize(g_rt->shared);
         (void)hipStreamDestroy(g_rt->shared);
+    }
+    {
+        auto &C = g_rt->cache;
+        for (hipStream_t st : {C.up, C.down})
+            if (st) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(st);
+            }
+        for (auto &sp : C.slots) {
+            sp->buf.release();
+            for (hipEvent_t ev : {sp->ready, sp->t0, sp->t1})
+                if (ev) (void)hipEventDestroy(ev);
+        }
+        for (int k = 0; k < 2; k++) {
+            C.stage[k].release();
+            if (C.stage_done[k]) (void)hipEventDestroy(C.stage_done[k]);
+        }
     }
     delete g_rt;
     g_rt = nullptr;
@@ -976,4 +1217,23 @@ extern "C" int nf_autotune_table(char *buf, size_t len) {
     const std::string out = os.str();
     if (buf && len) std::snprintf(buf, len, "%s", out.c_str());
     return (int)out.size();
+}
+
+extern "C" int nf_stream_begin(void) {
+    std::lock_guard<std::mutex> lk(g_rt_mutex);
+    if (!g_rt) return fail(NF_ERR_STATE, "nf_stream_begin before nf_setup");
+    std::lock_guard<std::mutex> ck(g_rt->cache.m);
+    g_rt->cache.active = true;
+    g_rt->cache.epoch++;
+    return NF_OK;
+}
+
+extern "C" int nf_stream_end(void) {
+    std::lock_guard<std::mutex> lk(g_rt_mutex);
+    if (!g_rt) return fail(NF_ERR_STATE, "nf_stream_end before nf_setup");
+    std::lock_guard<std::mutex> ck(g_rt->cache.m);
+    g_rt->cache.active = false;
+    for (auto &sp : g_rt->cache.slots)
+        if (sp->refs == 0) sp->valid = false;  // slabs in use are dropped on release
+    return NF_OK;
 }

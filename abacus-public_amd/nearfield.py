@@ -71,7 +71,9 @@ class nf_task_stats(C.Structure):
         ("bytes_to_device", C.c_uint64), ("bytes_from_device", C.c_uint64),
         ("direct_total", C.c_uint64), ("padded_direct_total", C.c_uint64),
         ("assigned_buffer", C.c_int32), ("path", C.c_int32), ("group_size", C.c_int32),
-        ("group_direct_kernel_s", C.c_double), ("group_direct_total", C.c_uint64),
+        ("group_direct_kernel_s", C.c_double), ("group_direct_total", C.c_uint64), ("graph", C.c_int32),
+        ("upload_s", C.c_double), ("download_s", C.c_double), ("slabs_uploaded", C.c_int32),
+        ("slabs_reused", C.c_int32),
     ]
 
 
@@ -80,7 +82,7 @@ class nf_setup_params(C.Structure):
         ("cpd", C.c_int32), ("max_buffer_bytes", C.c_uint64), ("n_gpu", C.c_int32),
         ("buffers_per_device", C.c_int32), ("use_pinned", C.c_int32), ("path", C.c_int32),
         ("sinks_per_lane", C.c_int32), ("source_tile", C.c_int32), ("gpu_streams", C.c_int32),
-        ("max_group", C.c_int32),
+        ("max_group", C.c_int32), ("use_graphs", C.c_int32),
     ]
 
 
@@ -110,6 +112,8 @@ def load_library(path=LIB_PATH):
     lib.nf_device_copy.argtypes = [C.c_void_p, C.c_void_p, U64, I32]
     lib.nf_device_synchronize.argtypes = []
     lib.nf_autotune_table.argtypes = [C.c_char_p, C.c_size_t]
+    lib.nf_stream_begin.argtypes = []
+    lib.nf_stream_end.argtypes = []
     return lib
 
 
@@ -271,7 +275,7 @@ class NearFieldDriver:
 
     def __init__(self, cpd, softening_length, b2, nfradius=2, softening="single_spline",
                  path="batched", buffers_per_device=3, max_buffer_bytes=None,
-                 sinks_per_lane=0, gpu_streams=0, max_group=0):
+                 sinks_per_lane=0, gpu_streams=0, max_group=0, use_graphs=0):
         from synth import eps_param
         if NearFieldDriver._active is not None:
             raise NFError("one NearFieldDriver per process (the GPU module is a singleton)")
@@ -288,7 +292,7 @@ class NearFieldDriver:
         p = nf_setup_params(cpd=cpd, max_buffer_bytes=int(max_buffer_bytes), n_gpu=1,
                             buffers_per_device=buffers_per_device, use_pinned=-1, path=PATHS[path],
                             sinks_per_lane=sinks_per_lane, source_tile=0, gpu_streams=gpu_streams,
-                            max_group=max_group)
+                            max_group=max_group, use_graphs=use_graphs)
         msb, mrb = C.c_int32(0), C.c_int32(0)
         check(lib().nf_setup(C.byref(p), C.byref(msb), C.byref(mrb)), "nf_setup")
         NearFieldDriver._active = self
@@ -303,8 +307,15 @@ class NearFieldDriver:
 
     def submit(self, sic, blocking=False):
         h = C.c_void_p()
-        check(lib().nf_submit(C.byref(sic.desc), 1 if blocking else 0, C.byref(h)), "nf_submit")
-        return Task(h, sic)
+        check(lib().nf_submit(C.byref(sic.desc), 0, C.byref(h)), "nf_submit")
+        t = Task(h, sic)
+        if blocking:
+            try:
+                t.wait()
+            except NFError:
+                t.release()
+                raise
+        return t
 
     def execute_slab(self, box, slab, acc, blocking=False, nsplit=1):
         """ExecuteSlab for a host-resident box: plan the slab as ``nsplit`` SICs
@@ -332,6 +343,13 @@ class NearFieldDriver:
             stats.append(t.stats())
             t.release()
         return stats
+
+    def stream_begin(self):
+        """Host slabs stay unchanged until stream_end: reuse their HBM copies."""
+        check(lib().nf_stream_begin(), "nf_stream_begin")
+
+    def stream_end(self):
+        check(lib().nf_stream_end(), "nf_stream_end")
 
     def autotune_table(self):
         """The autotuner's table: occupancy histogram -> chosen tile shape."""

@@ -138,6 +138,11 @@ typedef struct nf_task_stats {
     int32_t group_size;            /* SICs that shared this task's persistent launch */
     double group_direct_kernel_s;  /* GPU time of that shared direct-kernel launch */
     uint64_t group_direct_total;   /* interactions of the whole group */
+    int32_t graph;                 /* 1 if the group ran as a captured HIP graph */
+    double upload_s;               /* GPU time of this task's share of slab uploads (host mode) */
+    double download_s;             /* GPU time of its acceleration copy-back (host mode) */
+    int32_t slabs_uploaded;        /* slabs this task's group uploaded (cache misses) */
+    int32_t slabs_reused;          /* slabs it found resident (cache hits) */
 } nf_task_stats;
 
 typedef struct nf_task nf_task;
@@ -158,6 +163,11 @@ typedef struct nf_setup_params {
                                      the workers still prepare them in parallel) */
     int32_t max_group;            /* batched path: SICs queued together may share one
                                      persistent launch, up to this many (0: 8) */
+    int32_t use_graphs;           /* 1: each group's launch sequence (plan upload,
+                                     gathers, direct kernel, co-adds) is captured in a
+                                     HIP graph and launched as one; a worker keeps its
+                                     executable graph and updates it in place when the
+                                     next group has the same shape */
 } nf_setup_params;
 
 /* GetNGPU (GPUControl.cu:267-271). */
@@ -205,6 +215,20 @@ const char *nf_last_error(void);
  * later group with that histogram.  Writes the table as JSON into buf;
  * returns the length it needs (excluding the terminator), or < 0. */
 int nf_autotune_table(char *buf, size_t len);
+
+/* ---- host-slab streaming ---------------------------------------------
+ * No counterpart in the reference.  Between nf_stream_begin and
+ * nf_stream_end the caller promises that the host PosXYZ slabs named by
+ * submitted SICs do not change (the reference's pipeline keeps a PosXYZ slab
+ * fixed while SICs that read it are pending).  Inside such a session the
+ * module keeps recently used host slabs in HBM keyed by address, so
+ * consecutive slab SICs upload each slab once; uploads and acceleration
+ * copy-backs run on a copy stream through double-buffered pinned staging
+ * and overlap other groups' kernels.  Outside a session every group uploads
+ * its slabs afresh.  nf_stream_end waits for no task; slabs still in use are
+ * dropped when their tasks complete. */
+int nf_stream_begin(void);
+int nf_stream_end(void);
 
 /* ---- device memory for HBM-resident slabs ------------------------------
  * No counterpart in the reference, whose slabs live in host memory.  These

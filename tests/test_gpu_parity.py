@@ -203,3 +203,96 @@ def test_autotuner_keys_on_occupancy(box9, ref9):
         ref, soft = ref9[x]
         st = summarize(rel_err(accs[x], ref), not_soft_dominated(ref, soft))
         assert st["rms"] < 1e-6 and st["max"] < 1e-5, (x, st)
+
+
+def test_graph_replays_match_eager(box9):
+    """Groups captured as HIP graphs (re-captured, updated in place and
+    replayed on later groups) give bitwise the eager results, on every
+    replay, with eager device work and synchronisations between replays."""
+    eps, b2, _ = params(box9)
+    slabs = [2, 3, 4]
+    eager = run_box(box9, "batched", slabs=slabs, sinks_per_lane=2)
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                             max_buffer_bytes=2 << 30, sinks_per_lane=2, max_group=1, use_graphs=1) as drv:
+        for rep in range(3):
+            accs = {x: np.full((box9.slab(x).n, 4), np.nan, np.float32) for x in slabs}
+            for x in slabs:
+                drv.execute_slab(box9, x, accs[x], blocking=True)
+                st = drv.finalize(x)
+                assert all(s["graph"] == 1 for s in st)
+            for x in slabs:
+                assert np.array_equal(accs[x], eager[x]), (rep, x)
+            # eager device work and a synchronisation between replays
+            scratch = nfm.DeviceArray(host=np.arange(1 << 16, dtype=np.float32))
+            assert np.array_equal(scratch.to_host(np.float32, (1 << 16,)), np.arange(1 << 16, dtype=np.float32))
+            scratch.free()
+            nfm.device_synchronize()
+
+
+def test_inconsistent_descriptor_is_rejected(box9):
+    """A descriptor whose arrays disagree is refused before any GPU work
+    (error code, not a fault): block parents, counts and cell ranges."""
+    eps, b2, _ = params(box9)
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                             max_buffer_bytes=1 << 30) as drv:
+        acc = np.zeros((box9.slab(3).n, 4), np.float32)
+        for field, mutate in (("SinkBlockParentPencil", lambda a: a.__setitem__(0, a[-1])),
+                              ("SinkSetCount", lambda a: a.__setitem__(5, a[5] + 1)),
+                              ("SinkSourceInteractionList", lambda a: a.__setitem__(3, 10 ** 6))):
+            sic = drv.plan(box9, 3, acc_ptr=acc.ctypes.data)
+            n = {"SinkBlockParentPencil": sic.desc.NSinkBlocks, "SinkSetCount": sic.desc.NSinkSets,
+                 "SinkSourceInteractionList": sic.desc.InteractionCount}[field]
+            arr = np.ctypeslib.as_array((nfm.C.c_int32 * n).from_address(getattr(sic.desc, field))).copy()
+            mutate(arr)
+            setattr(sic.desc, field, arr.ctypes.data)
+            sic._keep = arr
+            with pytest.raises(nfm.NFError):
+                drv.submit(sic, blocking=True)
+        # the driver still works afterwards
+        t = drv.submit(drv.plan(box9, 3, acc_ptr=acc.ctypes.data), blocking=True)
+        t.release()
+        assert np.isfinite(acc).all()
+
+
+def test_stream_session_reuses_slabs(box9):
+    """Inside a stream session, host slabs already in HBM are reused by later
+    SICs (cache hits), results are bitwise those of uploads every time, and a
+    new session sees host data changed in between."""
+    eps, b2, _ = params(box9)
+    kw = dict(path="batched", buffers_per_device=2, max_buffer_bytes=2 << 30, sinks_per_lane=2, max_group=2)
+
+    def run(drv, box, slabs, session):
+        if session:
+            drv.stream_begin()
+        accs = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in slabs}
+        for x in slabs:
+            drv.execute_slab(box, x, accs[x], nsplit=2)
+        stats = [s for x in slabs for s in drv.finalize(x)]
+        if session:
+            drv.stream_end()
+        return accs, stats
+
+    slabs = list(range(box9.cpd))
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, **kw) as drv:
+        plain, st0 = run(drv, box9, slabs, False)
+        cached, st1 = run(drv, box9, slabs, True)
+    assert sum(s["slabs_reused"] for s in st0) == 0
+    up1 = sum(s["slabs_uploaded"] for s in st1)
+    assert sum(s["slabs_reused"] for s in st1) > 0 and up1 < sum(s["slabs_uploaded"] for s in st0)
+    assert up1 >= box9.cpd
+    assert all(s["upload_s"] >= 0 and s["download_s"] > 0 for s in st1)
+    for x in slabs:
+        assert np.array_equal(plain[x], cached[x]), x
+
+    # A second session after slab 4 changed on the host: its new positions
+    # are used (no stale copy), and the result equals a fresh driver's.
+    mod = synth.Box(box9.cpd, [synth.Slab(s.pos.copy(), s.count) for s in box9.slabs])
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, **kw) as drv:
+        first, _ = run(drv, mod, [3, 4], True)
+        mod.slab(4).pos *= np.float32(0.5)          # stays inside each cell
+        second, _ = run(drv, mod, [3, 4], True)
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, **kw) as drv:
+        fresh, _ = run(drv, mod, [3, 4], False)
+    for x in (3, 4):
+        assert not np.array_equal(first[x], second[x]), x
+        assert np.array_equal(second[x], fresh[x]), x
