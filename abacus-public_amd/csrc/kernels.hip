@@ -512,8 +512,17 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
     }
 }
 
-__device__ __forceinline__ void wave_two_sum(float &h, float &l) {
-    for (int off = 32; off >= 1; off >>= 1) {
+// Short tail of a pencil (fewer than kTailLanes sinks): the wave is four
+// groups of 16 lanes; each group takes SU sinks of the pass (4*SU sinks per
+// pass) and its lanes stride over the sources 16 apart.  Per-lane partial
+// sums are TwoSum-reduced within each group by a 4-level butterfly, which
+// reduces the four groups' sinks at once.
+constexpr int kGroupLanes = 16;
+constexpr int kGroups = kWave / kGroupLanes;
+
+// TwoSum butterfly within aligned groups of kGroupLanes lanes.
+__device__ __forceinline__ void group_two_sum(float &h, float &l) {
+    for (int off = kGroupLanes / 2; off >= 1; off >>= 1) {
         const float h2 = __shfl_xor(h, off);
         const float l2 = __shfl_xor(l, off);
         float s, e;
@@ -522,34 +531,35 @@ __device__ __forceinline__ void wave_two_sum(float &h, float &l) {
         l = l + l2 + e;
     }
 }
-__device__ __forceinline__ float wave_sum(float v) {
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+__device__ __forceinline__ float group_sum(float v) {
+    for (int off = kGroupLanes / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
-// Remainder of a pencil (fewer than 64*S sinks): sources on the lanes, SU
-// sinks wave-uniform per pass, per-lane partial sums reduced across the wave.
 template <int SOFT, int SU>
 __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane) {
-    constexpr int TC = 16;  // chunks of 64 sources per TwoSum tile
+    constexpr int TC = 16;  // 64-source blocks per TwoSum tile
     const NF_CONST float4 *sinks = cptr<float4>(v.sinks4);
-    const NF_CONST float4 *src = cptr<float4>(v.sources4);
     const NF_CONST float4 *box = cptr<float4>(v.box4);
     const NF_CONST int32_t *sink_set_start = cptr<int32_t>(v.sink_set_start);
     const NF_CONST int32_t *src_set_start = cptr<int32_t>(v.src_set_start);
     const NF_CONST int32_t *src_set_count = cptr<int32_t>(v.src_set_count);
     const NF_CONST int32_t *ilist = cptr<int32_t>(v.ilist);
+    const float4 *src = static_cast<const float4 *>(v.sources4);
     NF_GLOBAL double *out = (NF_GLOBAL double *)v.partial;
     const int s = w.sinkset;
     const int base = sink_set_start[s] + w.first;
     const int r = w.nsink;
     const int W = v.W;
-    for (int g = 0; g < r; g += SU) {
+    const int grp = lane / kGroupLanes, gl = lane % kGroupLanes;
+    for (int g0 = 0; g0 < r; g0 += kGroups * SU) {
+        // This group's sinks: g0 + grp*SU + u (repeats of the last sink past
+        // the tail are computed and not stored).
         float kx[SU], ky[SU], kz[SU];
         float blx = __builtin_huge_valf(), bly = blx, blz = blx, bhx = -blx, bhy = -blx, bhz = -blx;
 #pragma unroll
         for (int u = 0; u < SU; u++) {
-            const int gi = g + u < r ? g + u : r - 1;
+            const int gi = min(g0 + grp * SU + u, r - 1);
             const float4 k = sinks[base + gi];
             kx[u] = k.x;
             ky[u] = k.y;
@@ -561,6 +571,13 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
             bhy = fmaxf(bhy, k.y);
             bhz = fmaxf(bhz, k.z);
         }
+        // The pass's box over all four groups: the tile mode is wave-uniform.
+        blx = wave_min(blx);
+        bly = wave_min(bly);
+        blz = wave_min(blz);
+        bhx = wave_max(bhx);
+        bhy = wave_max(bhy);
+        bhz = wave_max(bhz);
         float hx[SU], hy[SU], hz[SU], lx[SU], ly[SU], lz[SU], hw[SU];
 #pragma unroll
         for (int u = 0; u < SU; u++) hx[u] = hy[u] = hz[u] = lx[u] = ly[u] = lz[u] = hw[u] = 0.f;
@@ -580,25 +597,31 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
 #pragma unroll
                 for (int u = 0; u < SU; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
                 for (int b = b0; b < b1; b++) {
-                    const float4 sv = src[st + b * kWave + lane];
                     const int64_t gb = (int64_t)st / kWave + b;
                     const int mode =
                         tile_mode<SOFT>(cx, box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]));
-                    if (mode == kNoDensity) {
+                    // The block's 64 sources (padding included: exact zeros),
+                    // 16 at a time on the group's lanes.
+                    const float4 *bp = src + st + b * kWave + gl;
 #pragma unroll
-                        for (int u = 0; u < SU; u++)
-                            pair_box<SOFT, kNoDensity>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u],
+                    for (int c = 0; c < kGroups; c++) {
+                        const float4 sv = bp[c * kGroupLanes];
+                        if (mode == kNoDensity) {
+#pragma unroll
+                            for (int u = 0; u < SU; u++)
+                                pair_box<SOFT, kNoDensity>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u],
+                                                           ty[u], tz[u], tw[u]);
+                        } else if (mode == kDensity) {
+#pragma unroll
+                            for (int u = 0; u < SU; u++)
+                                pair_box<SOFT, kDensity>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u],
+                                                         tz[u], tw[u]);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < SU; u++)
+                                pair_box<SOFT, kExact>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u],
                                                        tz[u], tw[u]);
-                    } else if (mode == kDensity) {
-#pragma unroll
-                        for (int u = 0; u < SU; u++)
-                            pair_box<SOFT, kDensity>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u],
-                                                     tz[u], tw[u]);
-                    } else {
-#pragma unroll
-                        for (int u = 0; u < SU; u++)
-                            pair_box<SOFT, kExact>(sv.x, sv.y, sv.z, kx[u], kyy[u], kz[u], cx.pc, tx[u], ty[u], tz[u],
-                                                   tw[u]);
+                        }
                     }
                 }
 #pragma unroll
@@ -612,12 +635,13 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
         }
 #pragma unroll
         for (int u = 0; u < SU; u++) {
-            wave_two_sum(hx[u], lx[u]);
-            wave_two_sum(hy[u], ly[u]);
-            wave_two_sum(hz[u], lz[u]);
-            const float ws = wave_sum(hw[u]);
-            if (lane == 0 && g + u < r) {
-                NF_GLOBAL double *o = out + 4 * (int64_t)(base + g + u);
+            group_two_sum(hx[u], lx[u]);
+            group_two_sum(hy[u], ly[u]);
+            group_two_sum(hz[u], lz[u]);
+            const float ws = group_sum(hw[u]);
+            const int gi = g0 + grp * SU + u;
+            if (gl == 0 && gi < r) {
+                NF_GLOBAL double *o = out + 4 * (int64_t)(base + gi);
                 o[0] = (double)hx[u] + (double)lx[u];
                 o[1] = (double)hy[u] + (double)ly[u];
                 o[2] = (double)hz[u] + (double)lz[u];
@@ -627,12 +651,28 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
     }
 }
 
-constexpr int kRemSinks = 4;
+constexpr int kRemSinks = 2;  // sinks per 16-lane group per pass of a short tail
 
 // Persistent kernel: every wave pops items (of any SIC of the group) until
 // the queue is empty.
-template <int SOFT, int S>
-__global__ __launch_bounds__(256) void k_batched(BatchCtx cx, const SicView *__restrict__ views,
+// Occupancy target WPE (waves per SIMD): more resident waves hide the v_rsq
+// and scalar-load latencies, as long as the registers fit without spills.
+// The most waves at which each (pair form, sinks per lane) compiles with no
+// VGPR spills for gfx950 (6 waves: 80 VGPRs; 5: 96; 4: 128; 2: 256); the
+// single spline and Plummer forms keep 4 waves at S = 4, which measured
+// faster than 5.
+constexpr int batched_waves_per_eu(int soft, int s) {
+    switch (soft) {
+        case NF_SOFT_SINGLE_SPLINE: return s == 4 ? 4 : 6;
+        case NF_SOFT_PLUMMER: return s == 4 ? 4 : 5;
+        case NF_SOFT_CUBIC_SPLINE: return s == 1 ? 5 : s == 2 ? 4 : 2;
+        default: return s == 4 ? 5 : 6;  // cubic Plummer
+    }
+}
+
+template <int SOFT, int S, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_batched(
+    BatchCtx cx, const SicView *__restrict__ views,
                                                  const WorkItem *__restrict__ items, int32_t nitems,
                                                  int32_t *__restrict__ counter) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -733,7 +773,7 @@ template <int SOFT, int S>
 static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const SicView *views, const WorkItem *items,
                             int32_t nitems, int32_t *counter, hipStream_t st) {
     int blocks_per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_batched<SOFT, S>, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_batched<SOFT, S, batched_waves_per_eu(SOFT, S)>, 256, 0) != hipSuccess ||
         blocks_per_cu <= 0)
         blocks_per_cu = 1;
     if (ts.waves_per_cu > 0) blocks_per_cu = std::max(1, std::min(blocks_per_cu, ts.waves_per_cu / 4));
@@ -742,7 +782,7 @@ static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const SicVi
     const int need = (nitems + 3) / 4;
     if (grid > need) grid = std::max(1, need);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL((k_batched<SOFT, S>), dim3(grid), dim3(256), 0, st, cx, views, items, nitems, counter);
+    hipLaunchKernelGGL((k_batched<SOFT, S, batched_waves_per_eu(SOFT, S)>), dim3(grid), dim3(256), 0, st, cx, views, items, nitems, counter);
     return (int)hipGetLastError();
 }
 

@@ -145,6 +145,7 @@ struct Worker {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t plan_ready = nullptr;  // the plan image upload (copy stream) is done
     hipGraphExec_t gexec = nullptr;  // the last group's executable graph (use_graphs)
     uint64_t graphs_instantiated = 0, graphs_updated = 0;
     DevBuf plan, work, slabs;
@@ -162,8 +163,11 @@ struct Runtime {
     std::mutex qm;
     std::condition_variable qcv;
     bool stopping = false;
+    int inflight = 0;   // groups taken by workers and not finished (under qm)
+    bool warm = false;  // work has been flowing since the queue was last empty (under qm)
     TileShape shape{2, 64, 4, 0};
     int max_group = 8;
+    uint64_t group_interactions = 12000000000ull;
     bool autotune = false;
     std::mutex tune_m;
     struct TuneEntry {
@@ -391,27 +395,33 @@ double event_ms(hipEvent_t a, hipEvent_t b) {
 
 // ---- Autotuner -------------------------------------------------------
 
-// Candidate tile shapes: sinks per lane x persistent waves per CU (0: the
-// occupancy limit).
-const TileShape kCandidates[] = {{1, 64, 4, 0}, {2, 64, 4, 0}, {4, 64, 4, 0}, {2, 64, 4, 8}, {4, 64, 4, 8}};
+// Candidate tile shapes (sinks per lane; persistent waves from occupancy).
+// Fewer resident waves (8 per CU) never won on uniform or clustered slabs,
+// so they are not tried.
+const TileShape kCandidates[] = {{1, 64, 4, 0}, {2, 64, 4, 0}, {4, 64, 4, 0}};
 constexpr int kNumCandidates = sizeof(kCandidates) / sizeof(kCandidates[0]);
 
-// Histogram of sink-pencil occupancies over log2 bins (0, 1, 2-3, 4-7, ...),
-// as fractions of the group's pencils quantised to 1/16: the tuning key.
+// The tuning key: a coarse histogram of sink-pencil occupancy (the fraction
+// of the group's sinks in pencils of <64, 64-255, 256-1023 and >=1024 sinks,
+// in quarters) and the group's size class (below 2^31, below 2^33, or more
+// pair interactions: small launches are limited by their longest items,
+// large ones by throughput).  Coarse on purpose: every new key costs one
+// launch per candidate.
 std::string occupancy_key(const std::vector<Task *> &g) {
-    constexpr int kBins = 16;
-    double h[kBins] = {0};
+    double h[4] = {0, 0, 0, 0};
     double tot = 0;
-    for (Task *t : g)
+    uint64_t di = 0;
+    for (Task *t : g) {
+        di += t->sic.DirectTotal;
         for (int k = 0; k < t->sic.NSinkSets; k++) {
             const int n = t->sic.SinkSetCount[k];
-            int b = 0;
-            while (b < kBins - 1 && (1 << b) <= n) b++;
-            h[b] += 1;
-            tot += 1;
+            h[n < 64 ? 0 : n < 256 ? 1 : n < 1024 ? 2 : 3] += n;
+            tot += n;
         }
+    }
     std::ostringstream os;
-    for (int b = 0; b < kBins; b++) os << (b ? "," : "") << (int)std::lround(16.0 * h[b] / std::max(tot, 1.0));
+    for (int b = 0; b < 4; b++) os << (b ? "," : "") << (int)std::lround(4.0 * h[b] / std::max(tot, 1.0));
+    os << ";" << (di < (1ull << 31) ? 0 : di < (1ull << 33) ? 1 : 2);
     return os.str();
 }
 
@@ -521,13 +531,24 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
 
     // Work list over the whole group, longest items first: one list per
     // distinct sinks-per-lane value to be launched (several while tuning).
+    // Items within 1/8 octave of cost count as equal and are kept in sink
+    // pencil order (SIC, z, y): pencils adjacent in y share 2R of their 2R+1
+    // source pencils.
     auto make_items = [&](int S) {
         std::vector<WorkItem> raw, out;
         std::vector<uint64_t> cost;
         for (int i = 0; i < n; i++) build_items(g[i]->sic, S, i, raw, cost);
+        std::vector<uint64_t> key(raw.size());
+        for (size_t i = 0; i < raw.size(); i++) {
+            const nf_sic &s = g[raw[i].kind_sic >> 2]->sic;
+            const uint32_t jj = (uint32_t)raw[i].sinkset / (uint32_t)s.Nk, kz = (uint32_t)raw[i].sinkset % (uint32_t)s.Nk;
+            const uint64_t bucket = cost[i] ? (uint64_t)std::floor(8.0 * std::log2((double)cost[i])) : 0;
+            key[i] = ((1023 - std::min<uint64_t>(bucket, 1023)) << 52) | ((uint64_t)(raw[i].kind_sic >> 2) << 40) |
+                     ((uint64_t)kz << 20) | jj;
+        }
         std::vector<size_t> order(raw.size());
         for (size_t i = 0; i < order.size(); i++) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
         out.resize(raw.size());
         for (size_t i = 0; i < order.size(); i++) out[i] = raw[order[i]];
         return out;
@@ -714,6 +735,11 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         if (!itemsets[k].empty())
             std::memcpy(hp + o_itemset[k], itemsets[k].data(), sizeof(WorkItem) * itemsets[k].size());
     const auto t_fill = clk::now();
+    // The plan image goes up on the copy stream, overlapping the kernels of
+    // the group before (this worker's previous group has completed, so its
+    // plan buffer is free).
+    NF_HIP(hipMemcpyAsync(w.plan.p, hp, plan_total, hipMemcpyHostToDevice, rt.cache.up));
+    NF_HIP(hipEventRecord(w.plan_ready, rt.cache.up));
 
     // ---- Enqueue the group as one uninterrupted sequence.
     hipStream_t stream = rt.shared ? rt.shared : w.stream;
@@ -732,8 +758,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             }
         }
     } capture{stream};
-    NF_HIP(hipEventRecord(w.ev[0], stream));
-    NF_HIP(hipMemcpyAsync(w.plan.p, hp, plan_total, hipMemcpyHostToDevice, stream));
+    NF_HIP(hipStreamWaitEvent(stream, w.plan_ready, 0));
     for (size_t k = 0; k < hslot.size(); k++) NF_HIP(hipStreamWaitEvent(stream, hslot[k]->ready, 0));
     for (size_t k = 0; k < hslab.size(); k++) to_dev += 12 * hslab_n[k];
     NF_HIP(hipEventRecord(w.ev[1], stream));
@@ -888,6 +913,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         st.download_s = download_ms * 1e-3 * share;
         st.slabs_uploaded = n_up;
         st.slabs_reused = n_reuse;
+        st.sinks_per_lane = path == NF_PATH_BATCHED ? ts.sinks_per_lane : 0;
+        st.autotune_trial = trial ? 1 : 0;
     }
     return NF_OK;
 }
@@ -911,16 +938,24 @@ void worker_main(Runtime *rt, Worker *w) {
         std::vector<Task *> g;
         {
             std::unique_lock<std::mutex> lk(rt->qm);
+            if (rt->queue.empty()) rt->warm = false;
             rt->qcv.wait(lk, [&] { return rt->stopping || !rt->queue.empty(); });
             if (rt->queue.empty()) return;  // stopping and drained
             g.push_back(rt->queue.front());
             rt->queue.pop_front();
             size_t bytes = task_bytes(g[0]->sic, rt->params.path);
-            while (rt->params.path == NF_PATH_BATCHED && (int)g.size() < rt->max_group && !rt->queue.empty() &&
-                   compatible(g[0]->sic, rt->queue.front()->sic)) {
+            // After the queue ran dry the GPU is idle: start it on a single SIC
+            // instead of waiting for a whole group to be prepared.
+            const int limit = !rt->warm && rt->inflight == 0 ? 1 : rt->max_group;
+            rt->warm = true;
+            rt->inflight++;
+            uint64_t work = g[0]->sic.DirectTotal;
+            while (rt->params.path == NF_PATH_BATCHED && (int)g.size() < limit && work < rt->group_interactions &&
+                   !rt->queue.empty() && compatible(g[0]->sic, rt->queue.front()->sic)) {
                 const size_t b = task_bytes(rt->queue.front()->sic, rt->params.path);
                 if (bytes + b > rt->max_buffer_bytes) break;
                 bytes += b;
+                work += rt->queue.front()->sic.DirectTotal;
                 g.push_back(rt->queue.front());
                 rt->queue.pop_front();
             }
@@ -932,8 +967,11 @@ void worker_main(Runtime *rt, Worker *w) {
             if (rc) finish(*t, rc);
             else ok.push_back(t);
         }
-        if (ok.empty()) continue;
-        const int rc = run_group(*rt, *w, ok);
+        const int rc = ok.empty() ? NF_OK : run_group(*rt, *w, ok);
+        {
+            std::lock_guard<std::mutex> lk(rt->qm);
+            rt->inflight--;
+        }
         for (Task *t : ok) finish(*t, rc);
     }
 }
@@ -942,6 +980,7 @@ int init_worker(Worker &w) {
     NF_HIP(hipSetDevice(w.device));
     NF_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
     for (auto &e : w.ev) NF_HIP(hipEventCreate(&e));
+    NF_HIP(hipEventCreateWithFlags(&w.plan_ready, hipEventDisableTiming));
     w.ready = true;
     return NF_OK;
 }
@@ -959,6 +998,7 @@ void destroy_worker(Worker &w) {
     w.gexec = nullptr;
     for (auto &e : w.ev)
         if (e) (void)hipEventDestroy(e);
+    if (w.plan_ready) (void)hipEventDestroy(w.plan_ready);
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.ready = false;
 }
@@ -1014,6 +1054,7 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
     rt->max_buffer_bytes = p->max_buffer_bytes;
     if (p->max_group < 0 || p->max_group > 4096) return fail(NF_ERR_INVALID, "max_group must be 0..4096");
     rt->max_group = p->max_group ? p->max_group : 8;
+    if (p->group_interactions) rt->group_interactions = p->group_interactions;
     if (p->sinks_per_lane != 0) {
         if (p->sinks_per_lane != 1 && p->sinks_per_lane != 2 && p->sinks_per_lane != 4)
             return fail(NF_ERR_INVALID, "sinks_per_lane must be 0, 1, 2 or 4");
@@ -1202,7 +1243,9 @@ extern "C" int nf_autotune_table(char *buf, size_t len) {
         bool first = true;
         for (const auto &kv : g_rt->tune) {
             const auto &e = kv.second;
-            os << (first ? "" : ",") << "{\"occupancy_hist_log2\":[" << kv.first << "],\"sinks_per_lane\":"
+            const size_t semi = kv.first.find(';');
+            os << (first ? "" : ",") << "{\"occupancy_quarters\":[" << kv.first.substr(0, semi)
+               << "],\"size_class\":" << kv.first.substr(semi + 1) << ",\"sinks_per_lane\":"
                << e.shape.sinks_per_lane << ",\"waves_per_cu\":" << e.shape.waves_per_cu << ",\"uses\":" << e.uses
                << ",\"interactions\":" << e.interactions << ",\"trial_ms\":[";
             for (size_t c = 0; c < e.trial_ms.size(); c++) os << (c ? "," : "") << e.trial_ms[c];

@@ -73,7 +73,7 @@ class nf_task_stats(C.Structure):
         ("assigned_buffer", C.c_int32), ("path", C.c_int32), ("group_size", C.c_int32),
         ("group_direct_kernel_s", C.c_double), ("group_direct_total", C.c_uint64), ("graph", C.c_int32),
         ("upload_s", C.c_double), ("download_s", C.c_double), ("slabs_uploaded", C.c_int32),
-        ("slabs_reused", C.c_int32),
+        ("slabs_reused", C.c_int32), ("sinks_per_lane", C.c_int32), ("autotune_trial", C.c_int32),
     ]
 
 
@@ -82,7 +82,7 @@ class nf_setup_params(C.Structure):
         ("cpd", C.c_int32), ("max_buffer_bytes", C.c_uint64), ("n_gpu", C.c_int32),
         ("buffers_per_device", C.c_int32), ("use_pinned", C.c_int32), ("path", C.c_int32),
         ("sinks_per_lane", C.c_int32), ("source_tile", C.c_int32), ("gpu_streams", C.c_int32),
-        ("max_group", C.c_int32), ("use_graphs", C.c_int32),
+        ("max_group", C.c_int32), ("use_graphs", C.c_int32), ("group_interactions", C.c_uint64),
     ]
 
 
@@ -275,7 +275,7 @@ class NearFieldDriver:
 
     def __init__(self, cpd, softening_length, b2, nfradius=2, softening="single_spline",
                  path="batched", buffers_per_device=3, max_buffer_bytes=None,
-                 sinks_per_lane=0, gpu_streams=0, max_group=0, use_graphs=0):
+                 sinks_per_lane=0, gpu_streams=0, max_group=0, use_graphs=0, group_interactions=0):
         from synth import eps_param
         if NearFieldDriver._active is not None:
             raise NFError("one NearFieldDriver per process (the GPU module is a singleton)")
@@ -292,7 +292,8 @@ class NearFieldDriver:
         p = nf_setup_params(cpd=cpd, max_buffer_bytes=int(max_buffer_bytes), n_gpu=1,
                             buffers_per_device=buffers_per_device, use_pinned=-1, path=PATHS[path],
                             sinks_per_lane=sinks_per_lane, source_tile=0, gpu_streams=gpu_streams,
-                            max_group=max_group, use_graphs=use_graphs)
+                            max_group=max_group, use_graphs=use_graphs,
+                            group_interactions=int(group_interactions))
         msb, mrb = C.c_int32(0), C.c_int32(0)
         check(lib().nf_setup(C.byref(p), C.byref(msb), C.byref(mrb)), "nf_setup")
         NearFieldDriver._active = self

@@ -13,6 +13,8 @@ Workloads:
   * ``clustered_box`` 70% lattice points displaced by a Gaussian random field
     with P(k) ~ k^-2 (rms displacement 0.5 cell) plus 30% in Plummer spheres
     whose masses follow dN/dM ~ M^-1.9, scale radii 0.02-0.3 cell widths.
+  * ``clustered_slabs`` the same recipe at production size (CPD 255, ~1e9
+    particles), realised only in the slabs a run needs.
 """
 import numpy as np
 
@@ -75,18 +77,19 @@ def box_from_global(pos, cpd, name=""):
     cell = np.floor((pos + 0.5) * cpd).astype(np.int64)     # Position2Cell, grid.cpp:133-137
     cell = np.clip(cell, 0, cpd - 1)
     local = (pos - cell_centre(cell, cpd)).astype(np.float32)
-    cid = (cell[:, 0] * cpd + cell[:, 1]) * cpd + cell[:, 2]
-    order = np.argsort(cid, kind="stable")
-    cid = cid[order]
-    local = local[order]
-    counts = np.bincount(cid, minlength=cpd ** 3).reshape(cpd, cpd * cpd)
+    # Bin by slab (a radix sort on the small slab index), then by cell within
+    # each slab; stable, so particles keep their input order inside a cell.
+    sx = cell[:, 0].astype(np.uint16)
+    order = np.argsort(sx, kind="stable")
+    bounds = np.concatenate([[0], np.cumsum(np.bincount(sx, minlength=cpd))])
     slabs = []
-    off = 0
     for x in range(cpd):
-        n = int(counts[x].sum())
-        xyz = local[off:off + n]
-        slabs.append(Slab(np.concatenate([xyz[:, 0], xyz[:, 1], xyz[:, 2]]), counts[x]))
-        off += n
+        idx = order[bounds[x]:bounds[x + 1]]
+        cyz = cell[idx, 1] * cpd + cell[idx, 2]
+        o2 = np.argsort(cyz, kind="stable")
+        xyz = local[idx[o2]]
+        cnt = np.bincount(cyz, minlength=cpd * cpd).astype(np.int32)
+        slabs.append(Slab(np.concatenate([xyz[:, 0], xyz[:, 1], xyz[:, 2]]), cnt))
     return Box(cpd, slabs, name)
 
 
@@ -170,6 +173,125 @@ def clustered_box(n1d, cpd, seed, halo_frac=0.3, n_halos=2000, rmin=0.02, rmax=0
     halo = centres[hc] + r[:, None] * v
     pos = np.concatenate([lat, halo])
     return box_from_global(pos, cpd, f"clustered n1d={n1d} cpd={cpd} seed={seed}")
+
+
+def _grf_mesh(ng, rms, rng):
+    """_grf_displacement on an ng^3 mesh, as float32 (ng, ng, ng, 3)."""
+    return _grf_displacement(ng, rms, rng).astype(np.float32).reshape(ng, ng, ng, 3)
+
+
+def _plane_displacement(mesh, x, n1d):
+    """Trilinear interpolation of the mesh on the whole lattice plane at global
+    x: the plane's (n1d, n1d, 3) displacements (lattice points at
+    ((j, k) + 0.5)/n1d - 0.5 in y and z)."""
+    ng = mesh.shape[0]
+
+    def axis_weights(q):
+        u = (np.asarray(q, np.float64) + 0.5) * ng - 0.5
+        i0 = np.floor(u).astype(np.int64)
+        f = (u - i0).astype(np.float32)
+        return i0 % ng, (i0 + 1) % ng, f
+
+    ix0, ix1, fx = axis_weights(x)
+    sl = (1.0 - fx) * mesh[ix0] + fx * mesh[ix1]                        # (ng, ng, 3) over (y, z)
+    q = (np.arange(n1d) + 0.5) / n1d - 0.5
+    i0, i1, f = axis_weights(q)
+    sl = (1.0 - f)[:, None, None] * sl[i0] + f[:, None, None] * sl[i1]    # (n1d, ng, 3)
+    return (1.0 - f)[None, :, None] * sl[:, i0] + f[None, :, None] * sl[:, i1]
+
+
+def _trilinear(mesh, q):
+    """Periodic trilinear interpolation of a cell-centred mesh (value (i,j,k)
+    at ((i,j,k) + 0.5)/ng - 0.5) at global positions q (n, 3)."""
+    ng = mesh.shape[0]
+    u = (q + 0.5) * ng - 0.5
+    i0 = np.floor(u).astype(np.int64)
+    f = (u - i0).astype(np.float32)
+    out = np.zeros((q.shape[0], 3), np.float32)
+    for dx in (0, 1):
+        wx = f[:, 0] if dx else 1.0 - f[:, 0]
+        ix = (i0[:, 0] + dx) % ng
+        for dy in (0, 1):
+            wy = f[:, 1] if dy else 1.0 - f[:, 1]
+            iy = (i0[:, 1] + dy) % ng
+            for dz in (0, 1):
+                wz = f[:, 2] if dz else 1.0 - f[:, 2]
+                iz = (i0[:, 2] + dz) % ng
+                out += (wx * wy * wz)[:, None] * mesh[ix, iy, iz]
+    return out
+
+
+def _in_slabs(x_lo, x_hi, cpd, wanted):
+    """Whether the x interval [x_lo, x_hi] (global, may leave the box) touches
+    any slab in the boolean array ``wanted``."""
+    c0 = int(np.floor((x_lo + 0.5) * cpd))
+    c1 = int(np.floor((x_hi + 0.5) * cpd))
+    if c1 - c0 + 1 >= cpd:
+        return bool(wanted.any())
+    return bool(wanted[np.arange(c0, c1 + 1) % cpd].any())
+
+
+def clustered_slabs(n1d, cpd, seed, slabs, grf_mesh=256, halos_per_cell=2000 / 69 ** 3, halo_frac=0.3,
+                    rmin=0.02, rmax=0.3, mass_slope=1.9, mass_range=4.0, rtrunc=10.0, lattice_frac=0.7):
+    """The clustered recipe of ``clustered_box`` at any size, realised only in
+    the listed slabs (every other slab of the returned Box is empty).
+
+    Every part of the realisation has its own seed, so the particles of a slab
+    do not depend on which other slabs are asked for:
+      * lattice plane i (x index) keeps each of its n1d^2 points with
+        probability ``lattice_frac`` (seed (seed, 1, i)) and displaces them by
+        a P(k) ~ k^-2 Gaussian random field of rms 0.5 cell, computed on a
+        ``grf_mesh``^3 mesh (seed (seed, 0)) and interpolated trilinearly;
+      * ``halos_per_cell * cpd^3`` Plummer spheres (2,000 at CPD 69) hold
+        ``halo_frac`` of n1d^3 particles; their masses (dN/dM ~ M^-1.9 on
+        [M0, mass_range M0]), centres and scale radii (log-uniform in
+        [rmin, rmax] cells) come from seed (seed, 2), and halo h's particles
+        from seed (seed, 3, h).
+    The nominal particle count (for the mean spacing) is n1d^3."""
+    w = 1.0 / cpd
+    wanted = np.zeros(cpd, bool)
+    wanted[np.asarray(list(slabs)) % cpd] = True
+    mesh = _grf_mesh(grf_mesh, 0.5 * w, np.random.default_rng([seed, 0]))
+    maxd = float(np.abs(mesh).max())
+    parts = []
+    ax = (np.arange(n1d) + 0.5) / n1d - 0.5
+    gy, gz = np.meshgrid(ax, ax, indexing="ij")
+    gy, gz = gy.ravel(), gz.ravel()
+    for i in range(n1d):
+        if not _in_slabs(ax[i] - maxd, ax[i] + maxd, cpd, wanted):
+            continue
+        rng = np.random.default_rng([seed, 1, i])
+        keep = rng.random(n1d * n1d) < lattice_frac
+        d = _plane_displacement(mesh, ax[i], n1d).reshape(-1, 3)[keep]
+        parts.append(np.stack([ax[i] + d[:, 0], gy[keep] + d[:, 1], gz[keep] + d[:, 2]], axis=1))
+    rng = np.random.default_rng([seed, 2])
+    n_halos = max(1, int(round(halos_per_cell * cpd ** 3)))
+    n_halo_parts = int(round(halo_frac * n1d ** 3))
+    a = 1.0 - mass_slope
+    u = rng.random(n_halos)
+    m = (1.0 + u * (float(mass_range) ** a - 1.0)) ** (1.0 / a)
+    nh = np.floor(m / m.sum() * n_halo_parts).astype(np.int64)
+    nh[np.argsort(-m)[:n_halo_parts - int(nh.sum())]] += 1
+    centres = rng.random((n_halos, 3)) - 0.5
+    radii = np.exp(rng.uniform(np.log(rmin), np.log(rmax), n_halos)) * w
+    xmax = rtrunc ** 3 / (rtrunc ** 2 + 1.0) ** 1.5
+    for h in range(n_halos):
+        ext = rtrunc * radii[h]
+        if nh[h] == 0 or not _in_slabs(centres[h, 0] - ext, centres[h, 0] + ext, cpd, wanted):
+            continue
+        hr = np.random.default_rng([seed, 3, h])
+        X = hr.random(nh[h]) * xmax
+        r = radii[h] / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
+        v = hr.standard_normal((nh[h], 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        parts.append(centres[h] + r[:, None] * v)
+    pos = np.concatenate(parts) if parts else np.zeros((0, 3))
+    pos = pos - np.floor(pos + 0.5)
+    sx = np.clip(np.floor((pos[:, 0] + 0.5) * cpd).astype(np.int64), 0, cpd - 1)
+    pos = pos[wanted[sx]]
+    box = box_from_global(pos, cpd, f"clustered n1d={n1d} cpd={cpd} seed={seed} slabs {int(wanted.sum())}/{cpd}")
+    box.N = n1d ** 3
+    return box
 
 
 # --- softening / FOF parameters -------------------------------------------

@@ -41,6 +41,16 @@ WORKLOADS = {
 }
 
 
+# Host-resident slab streams: (builder(slabs), cpd, n1d, description).  The
+# clustered recipe at CPD 255 and ~1e9 particles, realised in the slabs a run
+# of --stream-slabs sink slabs needs (plus the R-slab halo on each side).
+STREAM_WORKLOADS = {
+    "slabstream255": (lambda slabs: synth.clustered_slabs(998, 255, 20260255, slabs), 255, 998,
+                      "clustered N=998^3 CPD=255 (70% GRF lattice, 30% in 100,937 Plummer halos), "
+                      "host-resident slabs streamed through a 5-slab window"),
+}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -50,7 +60,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="uniform512", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="uniform512", choices=sorted(WORKLOADS) + sorted(STREAM_WORKLOADS))
+    ap.add_argument("--stream-slabs", type=int, default=40, help="slab-stream workloads: sink slabs per pass")
+    ap.add_argument("--graphs", type=int, default=None,
+                    help="capture each group's kernels in a HIP graph (default: on for slab streams)")
     ap.add_argument("--paths", default="plain,batched")
     ap.add_argument("--bpd", type=int, default=4, help="worker streams (DirectBPD)")
     ap.add_argument("--sinks-per-lane", type=int, default=0, help="0: the autotuner picks per occupancy histogram")
@@ -61,14 +74,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device-to-device twice (counter calibration) and exit")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="only print the HBM traffic table from rocprofv3 --pmc databases under DIR and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print_pmc_summary(args.pmc_summary, args.workload, args.calibrate_copy_mib or 1024)
+        return
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard by y-rows of each slab, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard each slab's y-rows across GPUs, with a 2R-row source halo)",
+              file=sys.stderr)
         sys.exit(2)
+    if args.workload in STREAM_WORKLOADS:
+        run_stream(args)
+        return
 
     build, cpd, desc = WORKLOADS[args.workload]
     t0 = time.time()
@@ -92,7 +114,8 @@ def main():
     for path in paths:
         with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path=path, buffers_per_device=args.bpd,
                                  max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
-                                 gpu_streams=args.gpu_streams, max_group=args.max_group) as drv:
+                                 gpu_streams=args.gpu_streams, max_group=args.max_group,
+                                 use_graphs=args.graphs or 0) as drv:
             tp = time.time()
             sics = []
             for x in range(cpd):
@@ -127,6 +150,17 @@ def main():
             tot = np.array([s["kernel_s"] for s in kstats])
             launches = float(sum(1.0 / s["group_size"] for s in kstats))
             per_slab_di = np.array([int(s.desc.DirectTotal) for s in sics] * args.steps, dtype=np.float64)
+            # Per direct-kernel instantiation (tile shape): launches, time and work.
+            shapes = {}
+            for st_, di_ in zip(kstats, per_slab_di):
+                e = shapes.setdefault(kernel_name(path, st_["sinks_per_lane"]),
+                                      dict(launches=0.0, kernel_s=0.0, interactions=0.0))
+                e["launches"] += 1.0 / st_["group_size"]
+                e["kernel_s"] += st_["direct_kernel_s"]
+                e["interactions"] += di_
+            for e in shapes.values():
+                e["avg_ms"] = 1e3 * e["kernel_s"] / e["launches"]
+                e["interactions_per_launch"] = e["interactions"] / e["launches"]
             results[path] = dict(
                 wall_s=wall, ms_per_step=1e3 * wall / args.steps,
                 interactions_per_s=direct_total * args.steps / wall,
@@ -137,9 +171,12 @@ def main():
                 launches_per_step=launches / args.steps,
                 interactions_per_launch=float(per_slab_di.sum() / launches),
                 direct_total=direct_total, padded_direct_total=padded_total, plan_s=plan_s,
+                kernels=shapes, n_sics=len(sics),
+                sic_sizes=dict(padded_sinks=float(np.mean([s.desc.PaddedSinkTotal for s in sics])),
+                               padded_sources=float(np.mean([s.desc.PaddedSourceTotal for s in sics]))),
             )
-            results[path]["acc_sample"] = [acc_d[x].to_host(np.float32, (box.slab(x).n, 4))[:2000]
-                                           for x in (0, cpd // 2)]
+            results[path]["acc_sample"] = {x: acc_d[x].to_host(np.float32, (box.slab(x).n, 4))[:2000]
+                                           for x in (0, cpd // 2)}
             if path == "batched":
                 results[path]["autotune"] = [{k: e[k] for k in ("sinks_per_lane", "waves_per_cu", "uses")}
                                              for e in drv.autotune_table()]
@@ -148,7 +185,10 @@ def main():
 
     best = max(results, key=lambda p: results[p]["interactions_per_s"])
     r = results[best]
-    achieved_tflops = FLOP_PER_INTERACTION * r["kernel_interactions_per_s"] / 1e12
+    # The dominant kernel: the direct-kernel instantiation with the most time.
+    kname, kd = max(r["kernels"].items(), key=lambda kv: kv[1]["kernel_s"])
+    achieved_tflops = FLOP_PER_INTERACTION * kd["interactions_per_launch"] / (kd["avg_ms"] * 1e-3) / 1e12
+    traffic, traffic_src = hbm_traffic(args.workload, kname)
     line = {
         "metric": METRIC,
         "value": r["interactions_per_s"],
@@ -166,10 +206,12 @@ def main():
                    "softening": "single_spline", "interactions_per_step": r["direct_total"]},
         "pct_fp32_peak": 100.0 * FLOP_PER_INTERACTION * r["interactions_per_s"] / (PEAK_FP32_TFLOPS * 1e12),
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_batched" if best == "batched" else "k_plain",
-                     "kernel_avg_ms": r["kernel_avg_ms"],
-                     "flop_per_interaction": FLOP_PER_INTERACTION},
+                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "kernel": kname, "kernel_avg_ms": kd["avg_ms"],
+                     "interactions_per_launch": kd["interactions_per_launch"],
+                     "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "algorithmic_bytes_per_launch": algorithmic_bytes(r, kd),
+                     "traffic_source": traffic_src},
         "paths": {p: {k: v for k, v in d.items() if k != "acc_sample"} for p, d in results.items()},
     }
     if "plain" in results and "batched" in results:
@@ -177,6 +219,233 @@ def main():
 
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(box, R, soft_len, args.cpu_seconds, results)
+    print(json.dumps(line), flush=True)
+
+
+def kernel_name(path, sinks_per_lane, softening=1):
+    """The direct kernel's name as rocprofv3 prints it (template arguments:
+    softening form, sinks per lane)."""
+    if path == "plain":
+        return f"k_plain<{softening}>"
+    return f"k_batched<{softening}, {sinks_per_lane}>"
+
+
+def algorithmic_bytes(r, kd):
+    """Bytes a batched launch must move at least: every padded sink and source
+    (float4) read once, two float4 bounding boxes per 64-source block, and a
+    float64x4 partial written per padded sink; for a launch's mean number of
+    SICs of mean size."""
+    sics_per_launch = kd["interactions_per_launch"] / (r["direct_total"] / r["n_sics"])
+    sz = r["sic_sizes"]
+    per_sic = 16 * sz["padded_sinks"] + 16 * sz["padded_sources"] * (1 + 2 / 64) + 32 * sz["padded_sinks"]
+    return per_sic * sics_per_launch
+
+
+def hbm_traffic(workload, kname):
+    """HBM bytes per launch of kernel kname, from the committed summary of the
+    two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this workload; None
+    when there is none."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 6 and cells[0] == kname:
+                return float(cells[5]), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def pmc_summary(prof_dir, workload):
+    """Markdown table of HBM bytes per dispatch from the rocprofv3 --pmc
+    databases under prof_dir: fetch_copy/write_copy (a device copy of known
+    size, run with --calibrate-copy-mib) give each counter's scale;
+    fetch_<workload>/write_<workload> give the kernels' counts."""
+    import glob
+    import sqlite3
+
+    def per_kernel(run, counter):
+        dbs = glob.glob(os.path.join(prof_dir, run, "*.db"))
+        if not dbs:
+            raise SystemExit(f"no database under {os.path.join(prof_dir, run)}")
+        out = {}
+        for db in dbs:
+            con = sqlite3.connect(db)
+            for name, n, avg in con.execute(
+                    "select kernel_name, count(*), avg(value) from counters_collection "
+                    "where counter_name = ? group by kernel_name", (counter,)):
+                short = name.replace("(anonymous namespace)", "").replace("void ", "").split("(")[0]
+                short = short.split("::")[-1].strip()
+                out[short] = (n, avg)
+            con.close()
+        return out
+
+    cal = {}
+    for counter, run in (("FETCH_SIZE", "fetch_copy"), ("WRITE_SIZE", "write_copy")):
+        k = per_kernel(run, counter)
+        name = max(k, key=lambda s: k[s][0] * k[s][1])
+        cal[counter] = (name, k[name][1])
+    fetch = per_kernel(f"fetch_{workload}", "FETCH_SIZE")
+    write = per_kernel(f"write_{workload}", "WRITE_SIZE")
+    return cal, fetch, write
+
+
+def sq_counters(prof_dir, run):
+    """{kernel: {counter: (dispatches, sum over dispatches)}} of one --pmc run."""
+    import glob
+    import sqlite3
+    out = {}
+    for db in glob.glob(os.path.join(prof_dir, run, "*.db")):
+        con = sqlite3.connect(db)
+        for name, counter, n, tot in con.execute(
+                "select kernel_name, counter_name, count(distinct dispatch_id), sum(value) "
+                "from counters_collection group by kernel_name, counter_name"):
+            short = name.replace("(anonymous namespace)", "").replace("void ", "").split("(")[0]
+            out.setdefault(short.split("::")[-1].strip(), {})[counter] = (n, tot)
+        con.close()
+    return out
+
+
+def print_valu_summary(prof_dir, workload):
+    """VALU instruction mix and busy fraction of the direct kernels, from the
+    valu_<workload> and busy_<workload> --pmc runs of a fixed-tile bench run
+    (no autotuner trials), per pair interaction of the run."""
+    runs = [r for r in (f"valu_{workload}", f"busy_{workload}") if os.path.isdir(os.path.join(prof_dir, r))]
+    if not runs:
+        return
+    with open(os.path.join(prof_dir, f"valu_{workload}.log")) as f:
+        line = json.loads([x for x in f if x.startswith("{")][-1])
+    interactions = line["config"]["interactions_per_step"] * (line["steps"] + line["warmup"])
+    print(f"\nVALU counters of the direct kernel over the whole run ({line['steps']} + {line['warmup']} steps, "
+          f"{interactions:.4e} pair interactions, tile fixed so every launch is the same kernel):\n")
+    print("| kernel | counter | dispatches | total | per interaction |")
+    print("|---|---|---|---|---|")
+    for run in runs:
+        for k, cs in sorted(sq_counters(prof_dir, run).items()):
+            if not k.startswith("k_batched") and not k.startswith("k_plain"):
+                continue
+            for c, (n, tot) in sorted(cs.items()):
+                per = ""
+                if c.startswith("SQ_INSTS_VALU") and "FLOPS" not in c:
+                    per = f"{64 * tot / interactions:.3f} lane-ops"
+                elif "FLOPS" in c or c in ("SQ_INSTS_SALU", "SQ_INSTS_SMEM"):
+                    per = f"{tot / interactions:.4f}" if "FLOPS" in c else f"{64 * tot / interactions:.4f} (x64)"
+                elif c in ("VALUBusy", "VALUUtilization"):
+                    per = f"{tot / max(n, 1):.1f} % (mean over dispatches)"
+                print(f"| {k} | {c} | {n} | {tot:.6g} | {per} |")
+
+
+def print_pmc_summary(prof_dir, workload, copy_mib):
+    print_valu_summary(prof_dir, workload)
+    if not os.path.isdir(os.path.join(prof_dir, "fetch_copy")):
+        return
+    cal, fetch, write = pmc_summary(prof_dir, workload)
+    copy_kib = copy_mib * 1024.0
+    f_scale = copy_kib / cal["FETCH_SIZE"][1]   # true KiB per counted KiB
+    w_scale = copy_kib / cal["WRITE_SIZE"][1]
+    print(f"Counter calibration: a {copy_mib} MiB device-to-device copy ({cal['FETCH_SIZE'][0]}) reads "
+          f"{cal['FETCH_SIZE'][1]:.0f} KiB of FETCH_SIZE and {cal['WRITE_SIZE'][1]:.0f} KiB of WRITE_SIZE per "
+          f"dispatch: scale {f_scale:.3f} (fetch), {w_scale:.3f} (write).\n")
+    print("| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | "
+          "HBM read+write MB/dispatch (scaled) | HBM bytes/dispatch (scaled) |")
+    print("|---|---|---|---|---|---|")
+    for k in sorted(set(fetch) | set(write), key=lambda s: -fetch.get(s, (0, 0))[1]):
+        nf, fa = fetch.get(k, (0, 0.0))
+        nw, wa = write.get(k, (0, 0.0))
+        b = 1024.0 * (fa * f_scale + wa * w_scale)
+        print(f"| {k} | {max(nf, nw)} | {fa:.1f} | {wa:.1f} | {b / 1e6:.2f} | {b:.0f} |")
+
+
+def run_stream(args):
+    """A slab stream: PosXYZ slabs and acceleration slabs in host memory, the
+    sink slabs submitted in order inside one stream session (each source slab
+    uploaded once while it is in the 2R+1 window), every launch group captured
+    in a HIP graph, tiles autotuned.  A step is one pass over the sink slabs;
+    the timed region includes every host-device transfer."""
+    build, cpd, n1d, desc = STREAM_WORKLOADS[args.workload]
+    R = 2
+    sinks = list(range(args.stream_slabs))
+    t0 = time.time()
+    box = build(range(-R, args.stream_slabs + R))
+    log(f"[bench] built {box.name}: {sum(s.n for s in box.slabs)} particles in {time.time() - t0:.1f}s")
+    ell = box.mean_spacing()
+    soft_len = ell / 40.0
+    b2 = synth.fof_b2(ell)
+    graphs = 1 if args.graphs is None else args.graphs
+    acc = {x: np.zeros((box.slab(x).n, 4), np.float32) for x in sinks}
+    with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path="batched", buffers_per_device=args.bpd,
+                             max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
+                             gpu_streams=args.gpu_streams, max_group=args.max_group, use_graphs=graphs) as drv:
+        tp = time.time()
+        sics = [drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST) for x in sinks]
+        plan_s = time.time() - tp
+        direct_total = sum(int(s.desc.DirectTotal) for s in sics)
+
+        def step():
+            drv.stream_begin()
+            tasks = [drv.submit(s) for s in sics]
+            st = []
+            for t in tasks:
+                t.wait()
+                st.append(t.stats())
+                t.release()
+            drv.stream_end()
+            return st
+
+        for i in range(args.warmup):
+            step()
+            log(f"[bench] stream: warmup {i + 1}/{args.warmup} done")
+        walls, stats = [], []
+        for i in range(args.steps):
+            t1 = time.perf_counter()
+            stats.extend(step())
+            walls.append(time.perf_counter() - t1)
+            log(f"[bench] stream: step {i + 1}/{args.steps} done ({walls[-1] * 1e3:.1f} ms)")
+        table = drv.autotune_table()
+    wall = float(sum(walls))
+    up = sum(x["upload_s"] for x in stats)
+    down = sum(x["download_s"] for x in stats)
+    busy = sum(x["kernel_s"] for x in stats)
+    kern = sum(x["direct_kernel_s"] for x in stats)
+    xfer = up + down
+    hidden = min(1.0, max(0.0, (busy + xfer - wall) / xfer)) if xfer > 0 else None
+    di = direct_total * args.steps
+    line = {
+        "metric": METRIC,
+        "value": di / wall,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": desc, "cpd": cpd, "N_nominal": n1d ** 3, "sink_slabs": len(sinks),
+                   "particles_in_window": int(sum(box.slab(x).n for x in range(-R, len(sinks) + R))),
+                   "interactions_per_step": direct_total, "path": "batched", "graphs": graphs,
+                   "softening": "single_spline", "mem": "host"},
+        "end_to_end": {
+            "interactions_per_s": di / wall,
+            "median_step_ms": 1e3 * float(np.median(walls)),
+            "device_busy_s_per_step": busy / args.steps,
+            "direct_kernel_interactions_per_s": di / kern if kern > 0 else None,
+            "h2d_s_per_step": up / args.steps, "d2h_s_per_step": down / args.steps,
+            "h2d_GB_per_step": sum(x["bytes_to_device"] for x in stats) / 1e9 / args.steps,
+            "d2h_GB_per_step": sum(x["bytes_from_device"] for x in stats) / 1e9 / args.steps,
+            "slabs_uploaded_per_step": sum(x["slabs_uploaded"] for x in stats) / args.steps,
+            "slabs_reused_per_step": sum(x["slabs_reused"] for x in stats) / args.steps,
+            "hidden_transfer_fraction": hidden,
+            "plan_s": plan_s,
+        },
+        "autotune": [{k: e[k] for k in ("occupancy_quarters", "size_class", "sinks_per_lane", "uses")}
+                     for e in table],
+    }
+    if not args.no_cpu:
+        results = {"batched": {"acc_sample": {x: acc[x][:2000] for x in (sinks[0], sinks[len(sinks) // 2])}}}
+        line["cpu_baseline"] = cpu_baseline(box, R, soft_len, args.cpu_seconds, results, slabs=sinks)
     print(json.dumps(line), flush=True)
 
 
@@ -192,7 +461,7 @@ def calibrate_copy(mib):
     print(json.dumps({"calibration_copy_bytes": n, "copies": 2}), flush=True)
 
 
-def cpu_baseline(box, R, soft_len, budget_s, results):
+def cpu_baseline(box, R, soft_len, budget_s, results, slabs=None):
     """The CPU-fallback restatement (oracle/cpu_directs.c, the reference's
     ExecuteSlabCPU loop + DirectCPUKernels.cc kernel, -Ofast AVX-512, OpenMP
     on all host cores), timed on whole slabs until the budget is used."""
@@ -200,7 +469,9 @@ def cpu_baseline(box, R, soft_len, budget_s, results):
     import pyoracle as po
     eps, _ = synth.eps_param("single_spline", soft_len)
     cpd = box.cpd
-    order = [0, cpd // 2] + [x for x in range(cpd) if x not in (0, cpd // 2)]
+    slabs = list(range(cpd)) if slabs is None else list(slabs)
+    first = list(next(iter(results.values()))["acc_sample"])
+    order = first + [x for x in slabs if x not in first]
     done = 0
     di = 0
     t0 = time.perf_counter()
@@ -209,10 +480,9 @@ def cpu_baseline(box, R, soft_len, budget_s, results):
         acc, n = po.execute_slab_cpu(box, R, "single_spline", eps, x)
         di += n
         done += 1
-        if x in (0, cpd // 2):
-            k = 0 if x == 0 else 1
+        if x in first:
             for p, d in results.items():
-                g = d["acc_sample"][k].astype(np.float64)
+                g = d["acc_sample"][x].astype(np.float64)
                 c = acc[:2000].astype(np.float64)
                 # the GPU carries the reference's output units (no e^-3), as does the CPU path
                 rel = np.linalg.norm(g[:, :3] - c[:, :3], axis=1) / np.linalg.norm(c[:, :3], axis=1)

@@ -143,6 +143,8 @@ typedef struct nf_task_stats {
     double download_s;             /* GPU time of its acceleration copy-back (host mode) */
     int32_t slabs_uploaded;        /* slabs this task's group uploaded (cache misses) */
     int32_t slabs_reused;          /* slabs it found resident (cache hits) */
+    int32_t sinks_per_lane;        /* batched path: register tile of its launch (0: plain) */
+    int32_t autotune_trial;        /* 1: its launch group timed every candidate tile */
 } nf_task_stats;
 
 typedef struct nf_task nf_task;
@@ -163,11 +165,14 @@ typedef struct nf_setup_params {
                                      the workers still prepare them in parallel) */
     int32_t max_group;            /* batched path: SICs queued together may share one
                                      persistent launch, up to this many (0: 8) */
-    int32_t use_graphs;           /* 1: each group's launch sequence (plan upload,
-                                     gathers, direct kernel, co-adds) is captured in a
+    int32_t use_graphs;           /* 1: each group's launch sequence (gathers, direct
+                                     kernel, co-adds) is captured in a
                                      HIP graph and launched as one; a worker keeps its
                                      executable graph and updates it in place when the
                                      next group has the same shape */
+    uint64_t group_interactions;  /* batched path: stop adding SICs to a launch group
+                                     once it holds this many pair interactions
+                                     (0: 1.2e10, about 4 ms of kernel time) */
 } nf_setup_params;
 
 /* GetNGPU (GPUControl.cu:267-271). */
@@ -209,10 +214,12 @@ int nf_release(nf_task *task);
 const char *nf_last_error(void);
 
 /* The batched-path autotuner's table (no counterpart in the reference).  With
- * sinks_per_lane = 0 the runtime keys every launch group on the histogram of
- * its sink-pencil occupancies; the first group with a new histogram times each
- * candidate tile shape on its own launch and the fastest is kept for every
- * later group with that histogram.  Writes the table as JSON into buf;
+ * sinks_per_lane = 0 the runtime keys every launch group on a coarse histogram
+ * of its sink-pencil occupancies (the share of its sinks in pencils of <64,
+ * 64-255, 256-1023 and >=1024 sinks, in quarters) and on its size class
+ * (<2^31, <2^33 or more pair interactions); the first group with a new key
+ * times each candidate tile shape on its own launch and the fastest is kept
+ * for every later group with that key.  Writes the table as JSON into buf;
  * returns the length it needs (excluding the terminator), or < 0. */
 int nf_autotune_table(char *buf, size_t len);
 

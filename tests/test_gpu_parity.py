@@ -199,6 +199,9 @@ def test_autotuner_keys_on_occupancy(box9, ref9):
     assert table and all(e["sinks_per_lane"] in (1, 2, 4) for e in table)
     assert sum(e["uses"] for e in table) == 4
     assert all(len(e["trial_ms"]) == len(e["candidates"]) for e in table)
+    # Every pencil holds 5 cells x ~20 sinks: one key, tuned once, reused.
+    assert len(table) == 1 and table[0]["uses"] == 4, table
+    assert table[0]["occupancy_quarters"] == [0, 4, 0, 0] and table[0]["size_class"] == 0
     for x in (0, 4, 8):
         ref, soft = ref9[x]
         st = summarize(rel_err(accs[x], ref), not_soft_dominated(ref, soft))
