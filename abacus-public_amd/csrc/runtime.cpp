@@ -187,21 +187,25 @@ struct Runtime {
         uint64_t n = 0;
         uint64_t epoch = 0;
         bool valid = false;
+        bool filling = false;  // an upload into it is being enqueued
         int refs = 0;
         uint64_t last = 0;
         DevBuf buf;
         hipEvent_t ready = nullptr;
         hipEvent_t t0 = nullptr, t1 = nullptr;  // brackets the last upload
     };
+    static constexpr int kStages = 4;  // pinned staging buffers for pageable host slabs
     struct SlabCache {
         std::mutex m;
+        std::condition_variable cv;  // a slot finished filling, or a staging buffer came free
         bool active = false;
         uint64_t epoch = 0, clock = 0;
         std::vector<std::unique_ptr<Slot>> slots;
         hipStream_t up = nullptr, down = nullptr;  // uploads; acceleration copy-backs
-        PinBuf stage[2];                           // double-buffered pinned staging
-        hipEvent_t stage_done[2] = {nullptr, nullptr};
-        int next = 0;
+        std::mutex up_enqueue;                     // keeps each upload's event bracket contiguous
+        PinBuf stage[kStages];
+        hipEvent_t stage_done[kStages] = {};
+        bool stage_busy[kStages] = {};
     } cache;
 };
 
@@ -427,22 +431,58 @@ std::string occupancy_key(const std::vector<Task *> &g) {
 
 // ---- Host-slab cache -------------------------------------------------
 
+// Whether host memory was allocated pinned by HIP (then DMA reads it directly).
+bool host_is_pinned(const void *p) {
+    hipPointerAttribute_t a;
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError();  // pageable memory reports an error; do not leave it behind
+    return e == hipSuccess && a.type == hipMemoryTypeHost;
+}
+
+// memcpy split over a few threads: one thread does not reach the host's
+// memory bandwidth, and a staged slab is tens of MB.
+void par_memcpy(void *dst, const void *src, size_t bytes) {
+    const size_t kMinChunk = 4u << 20;
+    unsigned nt = std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency() / 2));
+    nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, bytes / kMinChunk));
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t chunk = (bytes / nt + 63) & ~size_t(63);
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) {
+        const size_t o = t * chunk;
+        if (o >= bytes) break;
+        th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, std::min(chunk, bytes - o)); });
+    }
+    std::memcpy(dst, src, std::min(chunk, bytes));
+    for (auto &x : th) x.join();
+}
+
 // Find the HBM copy of a host slab (inside a stream session) or upload it
 // into the least recently used free slot.  Returns the slot with a reference
 // held; *uploaded says whether it was copied now (its t0/t1 events then
-// bracket the copy).  The upload is enqueued on the cache's upload stream
-// through pinned staging while the cache lock is held, so a second user of
-// the same slab finds its ready event recorded and waits on it on the GPU.
+// bracket the copy).  Pageable host memory goes through a pinned staging
+// buffer (copied by several threads); pinned memory is read by the DMA
+// directly.  The copy into staging runs outside the cache lock; a second user
+// of a slab being filled waits until its ready event is recorded, then waits
+// on that event on the GPU.
 int acquire_slab(Runtime &rt, const float *host, uint64_t n, Runtime::Slot **out, bool *uploaded) {
     auto &C = rt.cache;
-    std::lock_guard<std::mutex> lk(C.m);
+    std::unique_lock<std::mutex> lk(C.m);
     *uploaded = false;
     if (C.active)
         for (auto &sp : C.slots) {
             Runtime::Slot &sl = *sp;
-            if (sl.valid && sl.host == host && sl.n == n && sl.epoch == C.epoch) {
+            if ((sl.valid || sl.filling) && sl.host == host && sl.n == n && sl.epoch == C.epoch) {
                 sl.refs++;
                 sl.last = ++C.clock;
+                C.cv.wait(lk, [&] { return !sl.filling; });
+                if (!sl.valid) {
+                    sl.refs--;
+                    return fail(NF_ERR_HIP, "slab upload by another task failed");
+                }
                 *out = &sl;
                 return NF_OK;
             }
@@ -458,39 +498,59 @@ int acquire_slab(Runtime &rt, const float *host, uint64_t n, Runtime::Slot **out
         NF_HIP(hipEventCreate(&v->t1));
     }
     v->valid = false;
+    v->filling = true;
     v->refs = 1;
     v->last = ++C.clock;
     v->host = host;
     v->n = n;
     v->epoch = C.epoch;
     const size_t bytes = 12 * (size_t)n;
-    int rc = v->buf.ensure(bytes);
-    if (rc) {
-        v->refs = 0;
+    auto finish_fill = [&](int rc, int k) {  // under lk
+        v->filling = false;
+        v->valid = rc == NF_OK;
+        if (rc != NF_OK) v->refs = 0;
+        if (k >= 0) C.stage_busy[k] = false;
+        C.cv.notify_all();
         return rc;
+    };
+    int rc = v->buf.ensure(bytes);
+    if (rc) return finish_fill(rc, -1);
+    const bool direct = bytes == 0 || host_is_pinned(host);
+    int k = -1;
+    if (!direct) {
+        C.cv.wait(lk, [&] {
+            for (int i = 0; i < Runtime::kStages; i++)
+                if (!C.stage_busy[i]) return true;
+            return false;
+        });
+        for (int i = 0; i < Runtime::kStages && k < 0; i++)
+            if (!C.stage_busy[i]) k = i;
+        C.stage_busy[k] = true;
     }
-    if (bytes) {
-        const int k = C.next;
-        C.next ^= 1;
-        NF_HIP(hipEventSynchronize(C.stage_done[k]));  // the previous copy out of this buffer is done
-        if ((rc = C.stage[k].ensure(bytes))) {
-            v->refs = 0;
-            return rc;
+    lk.unlock();
+    const void *src = host;
+    hipError_t e = hipSuccess;
+    if (!direct) {
+        e = hipEventSynchronize(C.stage_done[k]);  // the last DMA out of this buffer is done
+        if (e == hipSuccess) rc = C.stage[k].ensure(bytes);
+        if (e == hipSuccess && rc == NF_OK) {
+            par_memcpy(C.stage[k].p, host, bytes);
+            src = C.stage[k].p;
         }
-        std::memcpy(C.stage[k].p, host, bytes);
-        NF_HIP(hipEventRecord(v->t0, C.up));
-        NF_HIP(hipMemcpyAsync(v->buf.p, C.stage[k].p, bytes, hipMemcpyHostToDevice, C.up));
-        NF_HIP(hipEventRecord(v->t1, C.up));
-        NF_HIP(hipEventRecord(C.stage_done[k], C.up));
-    } else {
-        NF_HIP(hipEventRecord(v->t0, C.up));
-        NF_HIP(hipEventRecord(v->t1, C.up));
     }
-    NF_HIP(hipEventRecord(v->ready, C.up));
-    v->valid = true;
-    *uploaded = true;
+    if (e == hipSuccess && rc == NF_OK) {
+        std::lock_guard<std::mutex> ul(C.up_enqueue);
+        e = hipEventRecord(v->t0, C.up);
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(v->buf.p, src, bytes, hipMemcpyHostToDevice, C.up);
+        if (e == hipSuccess) e = hipEventRecord(v->t1, C.up);
+        if (e == hipSuccess && k >= 0) e = hipEventRecord(C.stage_done[k], C.up);
+        if (e == hipSuccess) e = hipEventRecord(v->ready, C.up);
+    }
+    lk.lock();
+    if (e != hipSuccess) rc = fail(NF_ERR_HIP, std::string("slab upload: ") + hipGetErrorString(e));
+    if (rc == NF_OK) *uploaded = true;
     *out = v;
-    return NF_OK;
+    return finish_fill(rc, k);
 }
 
 void release_slab(Runtime &rt, Runtime::Slot *sl) {
@@ -760,7 +820,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     } capture{stream};
     NF_HIP(hipStreamWaitEvent(stream, w.plan_ready, 0));
     for (size_t k = 0; k < hslot.size(); k++) NF_HIP(hipStreamWaitEvent(stream, hslot[k]->ready, 0));
-    for (size_t k = 0; k < hslab.size(); k++) to_dev += 12 * hslab_n[k];
+    for (size_t k = 0; k < hslab.size(); k++)
+        if (fresh[k]) to_dev += 12 * hslab_n[k];
     NF_HIP(hipEventRecord(w.ev[1], stream));
     // The captured region holds kernels only; copies and timing events stay
     // outside it, around the graph launch.
@@ -873,17 +934,21 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             const nf_sink_plan &last = s.SinkPlan[(size_t)(s.j_width - 1) * s.Nk + (s.Nk - 1)];
             const uint64_t r1 = (uint64_t)last.cell[0].start + (uint64_t)last.cell[0].N;
             if (r1 <= r0) continue;
-            if ((rc = w.pin_acc.ensure(16 * (r1 - r0)))) return rc;
             // The copy-back runs on the download stream, after the group's
-            // kernels, while the next group's kernels use the compute stream.
+            // kernels, while the next group's kernels use the compute stream:
+            // straight into a pinned acceleration slab, else through staging.
+            float *dst = s.SinkAccSlab + 4 * r0;
+            const size_t nb = 16 * (size_t)(r1 - r0);
+            const bool direct = host_is_pinned(dst);
+            if (!direct && (rc = w.pin_acc.ensure(nb))) return rc;
             NF_HIP(hipEventRecord(w.ev[0], rt.cache.down));
-            NF_HIP(hipMemcpyAsync(w.pin_acc.p, accd[i] + 4 * r0, 16 * (r1 - r0), hipMemcpyDeviceToHost,
+            NF_HIP(hipMemcpyAsync(direct ? (void *)dst : w.pin_acc.p, accd[i] + 4 * r0, nb, hipMemcpyDeviceToHost,
                                   rt.cache.down));
             NF_HIP(hipEventRecord(w.ev[2], rt.cache.down));
             NF_HIP(hipEventSynchronize(w.ev[2]));
             download_ms += event_ms(w.ev[0], w.ev[2]);
-            std::memcpy(s.SinkAccSlab + 4 * r0, w.pin_acc.p, 16 * (r1 - r0));
-            from_dev += 16 * (r1 - r0);
+            if (!direct) par_memcpy(dst, w.pin_acc.p, nb);
+            from_dev += nb;
         }
         copyback = std::chrono::duration<double>(clk::now() - tc).count();
     }
@@ -1090,7 +1155,8 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
         auto &C = rt->cache;
         hipError_t e = hipStreamCreateWithFlags(&C.up, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&C.down, hipStreamNonBlocking);
-        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&C.stage_done[k], hipEventDisableTiming);
+        for (int k = 0; k < Runtime::kStages && e == hipSuccess; k++)
+            e = hipEventCreateWithFlags(&C.stage_done[k], hipEventDisableTiming);
         if (e != hipSuccess) {
             for (auto &x : rt->workers) destroy_worker(*x);
             return fail(NF_ERR_HIP, std::string("copy stream setup: ") + hipGetErrorString(e));
@@ -1138,7 +1204,7 @@ extern "C" int nf_reset(void) {
             for (hipEvent_t ev : {sp->ready, sp->t0, sp->t1})
                 if (ev) (void)hipEventDestroy(ev);
         }
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < Runtime::kStages; k++) {
             C.stage[k].release();
             if (C.stage_done[k]) (void)hipEventDestroy(C.stage_done[k]);
         }
@@ -1230,6 +1296,22 @@ extern "C" int nf_device_copy(void *dst, const void *src, uint64_t bytes, int32_
 
 extern "C" int nf_device_synchronize(void) {
     NF_HIP(hipDeviceSynchronize());
+    return NF_OK;
+}
+
+extern "C" int nf_host_alloc(uint64_t bytes, void **ptr) {
+    if (!ptr) return fail(NF_ERR_INVALID, "null argument");
+    *ptr = nullptr;
+    hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(NF_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    return NF_OK;
+}
+
+extern "C" int nf_host_free(void *ptr) {
+    if (ptr) NF_HIP(hipHostFree(ptr));
     return NF_OK;
 }
 

@@ -111,6 +111,8 @@ def load_library(path=LIB_PATH):
     lib.nf_device_free.argtypes = [C.c_void_p]
     lib.nf_device_copy.argtypes = [C.c_void_p, C.c_void_p, U64, I32]
     lib.nf_device_synchronize.argtypes = []
+    lib.nf_host_alloc.argtypes = [U64, C.POINTER(C.c_void_p)]
+    lib.nf_host_free.argtypes = [C.c_void_p]
     lib.nf_autotune_table.argtypes = [C.c_char_p, C.c_size_t]
     lib.nf_stream_begin.argtypes = []
     lib.nf_stream_end.argtypes = []
@@ -174,6 +176,33 @@ class DeviceArray:
     def free(self):
         if self.ptr:
             check(lib().nf_device_free(self.ptr), "nf_device_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """A numpy array in pinned host memory (nf_host_alloc): host-mode slabs the
+    DMA engines read and write directly.  ``.array`` is the numpy view; keep
+    this object alive while the view is in use."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(shape)) * dtype.itemsize
+        p = C.c_void_p()
+        check(lib().nf_host_alloc(self.nbytes, C.byref(p)), "nf_host_alloc")
+        self.ptr = p.value
+        buf = (C.c_char * max(self.nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            check(lib().nf_host_free(self.ptr), "nf_host_free")
             self.ptr = None
 
     def __del__(self):

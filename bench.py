@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="uniform512", choices=sorted(WORKLOADS) + sorted(STREAM_WORKLOADS))
     ap.add_argument("--stream-slabs", type=int, default=40, help="slab-stream workloads: sink slabs per pass")
+    ap.add_argument("--pinned", action="store_true",
+                    help="slab streams: host slabs in pinned memory (direct DMA) instead of pageable")
     ap.add_argument("--graphs", type=int, default=None,
                     help="capture each group's kernels in a HIP graph (default: on for slab streams)")
     ap.add_argument("--paths", default="plain,batched")
@@ -372,7 +374,20 @@ def run_stream(args):
     soft_len = ell / 40.0
     b2 = synth.fof_b2(ell)
     graphs = 1 if args.graphs is None else args.graphs
-    acc = {x: np.zeros((box.slab(x).n, 4), np.float32) for x in sinks}
+    keep = []
+
+    def host_array(shape, like=None):
+        if not args.pinned:
+            return np.zeros(shape, np.float32) if like is None else like
+        pa = nfm.PinnedArray(shape, np.float32)
+        keep.append(pa)
+        pa.array[:] = 0 if like is None else like
+        return pa.array
+
+    if args.pinned:
+        box = synth.Box(cpd, [synth.Slab(host_array(s.pos.shape, s.pos), s.count) for s in box.slabs], box.name)
+        box.N = n1d ** 3
+    acc = {x: host_array((box.slab(x).n, 4)) for x in sinks}
     with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path="batched", buffers_per_device=args.bpd,
                              max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
                              gpu_streams=args.gpu_streams, max_group=args.max_group, use_graphs=graphs) as drv:
@@ -426,7 +441,7 @@ def run_stream(args):
         "config": {"workload": desc, "cpd": cpd, "N_nominal": n1d ** 3, "sink_slabs": len(sinks),
                    "particles_in_window": int(sum(box.slab(x).n for x in range(-R, len(sinks) + R))),
                    "interactions_per_step": direct_total, "path": "batched", "graphs": graphs,
-                   "softening": "single_spline", "mem": "host"},
+                   "softening": "single_spline", "mem": "host pinned" if args.pinned else "host pageable"},
         "end_to_end": {
             "interactions_per_s": di / wall,
             "median_step_ms": 1e3 * float(np.median(walls)),

@@ -247,6 +247,11 @@ int nf_device_free(void *ptr);
 /* kind: 0 host->device, 1 device->host, 2 device->device (synchronous). */
 int nf_device_copy(void *dst, const void *src, uint64_t bytes, int32_t kind);
 int nf_device_synchronize(void);
+/* Pinned (page-locked) host memory.  Host-mode slabs and acceleration slabs
+ * allocated here are read and written by the DMA engines directly; pageable
+ * ones go through the module's pinned staging buffers. */
+int nf_host_alloc(uint64_t bytes, void **ptr);
+int nf_host_free(void *ptr);
 
 /* ---- planning (host) ------------------------------------------------- */
 

@@ -299,3 +299,42 @@ def test_stream_session_reuses_slabs(box9):
     for x in (3, 4):
         assert not np.array_equal(first[x], second[x]), x
         assert np.array_equal(second[x], fresh[x]), x
+
+
+def test_pinned_host_slabs_match_pageable(box9):
+    """Host-mode slabs and acceleration slabs in pinned memory (read and
+    written by the DMA engines directly) give bitwise the results of pageable
+    ones (copied through the pinned staging buffers)."""
+    eps, b2, _ = params(box9)
+    kw = dict(path="batched", buffers_per_device=2, max_buffer_bytes=2 << 30, sinks_per_lane=2)
+    keep = []
+    pinned_slabs = []
+    for s in box9.slabs:
+        pa = nfm.PinnedArray(s.pos.shape, np.float32)
+        pa.array[:] = s.pos
+        keep.append(pa)
+        pinned_slabs.append(synth.Slab(pa.array, s.count))
+    pbox = synth.Box(box9.cpd, pinned_slabs)
+    assert all(ps.pos.ctypes.data == pa.ptr for ps, pa in zip(pbox.slabs, keep))
+    slabs = [0, 4, 8]
+    out = {}
+    for name, box in (("pageable", box9), ("pinned", pbox)):
+        with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, **kw) as drv:
+            drv.stream_begin()
+            accs = {}
+            for x in slabs:
+                if name == "pinned":
+                    pa = nfm.PinnedArray((box.slab(x).n, 4), np.float32)
+                    pa.array[:] = np.nan
+                    keep.append(pa)
+                    accs[x] = pa.array
+                else:
+                    accs[x] = np.full((box.slab(x).n, 4), np.nan, np.float32)
+                drv.execute_slab(box, x, accs[x], nsplit=2)
+            for x in slabs:
+                drv.finalize(x)
+            drv.stream_end()
+            out[name] = {x: accs[x].copy() for x in slabs}
+    for x in slabs:
+        assert np.isfinite(out["pinned"][x]).all()
+        assert np.array_equal(out["pageable"][x], out["pinned"][x]), x
