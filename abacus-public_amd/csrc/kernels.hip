@@ -653,8 +653,6 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
 
 constexpr int kRemSinks = 2;  // sinks per 16-lane group per pass of a short tail
 
-// Persistent kernel: every wave pops items (of any SIC of the group) until
-// the queue is empty.
 // Occupancy target WPE (waves per SIMD): more resident waves hide the v_rsq
 // and scalar-load latencies, as long as the registers fit without spills.
 // The most waves at which each (pair form, sinks per lane) compiles with no
@@ -670,6 +668,16 @@ constexpr int batched_waves_per_eu(int soft, int s) {
     }
 }
 
+}  // namespace
+
+int batched_kernel_waves(int softening, int sinks_per_lane) {
+    return batched_waves_per_eu(softening, sinks_per_lane);
+}
+
+namespace {
+
+// Persistent kernel: every wave pops items (of any SIC of the group) until
+// the queue is empty.
 template <int SOFT, int S, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_batched(
     BatchCtx cx, const SicView *__restrict__ views,

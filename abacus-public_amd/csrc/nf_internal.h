@@ -105,6 +105,9 @@ inline void *box_after(void *sources4, int32_t nsrc_blocks) {
 int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
                      void *stream);
 int device_cu_count();
+// The occupancy target (waves per SIMD) the batched kernel is built with for
+// a pair form and tile; the third template argument of k_batched.
+int batched_kernel_waves(int softening, int sinks_per_lane);
 
 }  // namespace nf
 

@@ -980,6 +980,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         st.slabs_reused = n_reuse;
         st.sinks_per_lane = path == NF_PATH_BATCHED ? ts.sinks_per_lane : 0;
         st.autotune_trial = trial ? 1 : 0;
+        st.waves_per_eu = path == NF_PATH_BATCHED ? batched_kernel_waves(s0.softening, ts.sinks_per_lane) : 0;
     }
     return NF_OK;
 }

@@ -74,6 +74,7 @@ class nf_task_stats(C.Structure):
         ("group_direct_kernel_s", C.c_double), ("group_direct_total", C.c_uint64), ("graph", C.c_int32),
         ("upload_s", C.c_double), ("download_s", C.c_double), ("slabs_uploaded", C.c_int32),
         ("slabs_reused", C.c_int32), ("sinks_per_lane", C.c_int32), ("autotune_trial", C.c_int32),
+        ("waves_per_eu", C.c_int32),
     ]
 
 

@@ -155,7 +155,7 @@ def main():
             # Per direct-kernel instantiation (tile shape): launches, time and work.
             shapes = {}
             for st_, di_ in zip(kstats, per_slab_di):
-                e = shapes.setdefault(kernel_name(path, st_["sinks_per_lane"]),
+                e = shapes.setdefault(kernel_name(path, st_["sinks_per_lane"], st_["waves_per_eu"]),
                                       dict(launches=0.0, kernel_s=0.0, interactions=0.0))
                 e["launches"] += 1.0 / st_["group_size"]
                 e["kernel_s"] += st_["direct_kernel_s"]
@@ -224,12 +224,12 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def kernel_name(path, sinks_per_lane, softening=1):
+def kernel_name(path, sinks_per_lane, waves_per_eu=0, softening=1):
     """The direct kernel's name as rocprofv3 prints it (template arguments:
-    softening form, sinks per lane)."""
+    pair form, sinks per lane, occupancy target in waves per SIMD)."""
     if path == "plain":
         return f"k_plain<{softening}>"
-    return f"k_batched<{softening}, {sinks_per_lane}>"
+    return f"k_batched<{softening}, {sinks_per_lane}, {waves_per_eu}>"
 
 
 def algorithmic_bytes(r, kd):

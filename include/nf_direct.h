@@ -145,6 +145,7 @@ typedef struct nf_task_stats {
     int32_t slabs_reused;          /* slabs it found resident (cache hits) */
     int32_t sinks_per_lane;        /* batched path: register tile of its launch (0: plain) */
     int32_t autotune_trial;        /* 1: its launch group timed every candidate tile */
+    int32_t waves_per_eu;          /* batched path: occupancy target its kernel was built with */
 } nf_task_stats;
 
 typedef struct nf_task nf_task;
