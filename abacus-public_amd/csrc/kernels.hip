@@ -406,18 +406,31 @@ __device__ __forceinline__ int tile_mode(const BatchCtx &cx, float gap2) {
 }
 
 // One source tile (cnt8 <= 64 sources, a multiple of 8) against the S sinks
-// of every lane; sources are wave-uniform loads (scalar cache).
+// of every lane; sources are wave-uniform loads (scalar cache).  The force
+// terms of each batch of 8 sources are summed on their own before they join
+// the tile sum: a sum of 8 batch sums of 8 terms carries about 1.6x less
+// FP32 rounding than one running sum of 64 terms, for one add per component
+// per batch.
 template <int SOFT, int MODE, int S>
 __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, const float *kx, const float *ky,
                                           const float *kz, const PairConst &pc, float *tx, float *ty, float *tz,
                                           float *tw) {
     for (int i = 0; i < cnt8; i += 8) {
+        float bx[S], by[S], bz[S];
+#pragma unroll
+        for (int u = 0; u < S; u++) bx[u] = by[u] = bz[u] = 0.f;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const float4 sv = sp[i + q];
 #pragma unroll
             for (int u = 0; u < S; u++)
-                pair_box<SOFT, MODE>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, tx[u], ty[u], tz[u], tw[u]);
+                pair_box<SOFT, MODE>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, bx[u], by[u], bz[u], tw[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < S; u++) {
+            tx[u] += bx[u];
+            ty[u] += by[u];
+            tz[u] += bz[u];
         }
     }
 }

@@ -155,12 +155,14 @@ static inline int wrapi(int i, int cpd)
  * kernel sum in the reference's units (no eps_int^-3 prefactor; density
  * includes the self term b2, removed later by kick.cpp:105).  If soft_norm is
  * not NULL it receives |sum of the pair terms with r below the softening
- * length|, used to flag softening-dominated sinks. */
+ * length|, used to flag softening-dominated sinks.  If abs_norm is not NULL it
+ * receives sum_j |g_j| |d_j|, the sum of the pair terms' magnitudes: its ratio
+ * to |a| is the condition number of the sink's force sum. */
 void or_direct_f64(int cpd, const float *const *pos, const int64_t *nslab,
                    const int64_t *const *start, const int32_t *const *count, int R, int kind,
                    double eps, double b2, int64_t nsinks, const int32_t *sink_slab,
                    const int64_t *sink_index, const int32_t *sink_cell, double *out,
-                   double *soft_norm)
+                   double *soft_norm, double *abs_norm)
 {
     const double invcpd = 1.0 / (double)cpd;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -173,6 +175,7 @@ void or_direct_f64(int cpd, const float *const *pos, const int64_t *nslab,
         double kx = P[i], ky = P[n + i], kz = P[2 * n + i];
         double a[4] = {0, 0, 0, 0};
         double as[4] = {0, 0, 0, 0}; /* contribution of softened pairs (r < eps_int) */
+        double mag = 0.0;            /* sum of |pair term| */
         /* r^2 below which a pair is softened: single spline r < e (eps = 1/e^2),
          * Plummer r < e (eps = e^2), cubic spline u = r/e < 2 (eps = 1/e),
          * cubic Plummer r < e (eps = e^3). */
@@ -201,6 +204,11 @@ void or_direct_f64(int cpd, const float *const *pos, const int64_t *nslab,
                         double r2 = dx * dx + dy * dy + dz * dz;
                         if (soft_norm && r2 < soft_r2)
                             or_kernel_f64(kind, dx, dy, dz, eps, 0.0, as);
+                        if (abs_norm) {
+                            double t[4] = {0, 0, 0, 0};
+                            or_kernel_f64(kind, dx, dy, dz, eps, 0.0, t);
+                            mag += sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+                        }
                         or_kernel_f64(kind, dx, dy, dz, eps, b2, a);
                     }
                 }
@@ -211,6 +219,7 @@ void or_direct_f64(int cpd, const float *const *pos, const int64_t *nslab,
         out[4 * s + 2] = a[2];
         out[4 * s + 3] = a[3];
         if (soft_norm) soft_norm[s] = sqrt(as[0] * as[0] + as[1] * as[1] + as[2] * as[2]);
+        if (abs_norm) abs_norm[s] = mag;
     }
 }
 

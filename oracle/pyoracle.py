@@ -77,9 +77,11 @@ def pair_cpu_f32(kind, src, sink, eps):
 
 # ---- box-level oracles --------------------------------------------------------
 
-def direct_f64(box, R, kind, eps, b2, sink_slab, sink_index, sink_cell, with_soft=False):
-    """FP64 brute-force sum for the listed sinks -> (n, 4) float64 (and, with
-    ``with_soft``, the norm of the softened-pair part of each sum)."""
+def direct_f64(box, R, kind, eps, b2, sink_slab, sink_index, sink_cell, with_soft=False, with_abs=False):
+    """FP64 brute-force sum for the listed sinks -> (n, 4) float64; with
+    ``with_soft`` also the norm of the softened-pair part of each sum, with
+    ``with_abs`` also the sum of the pair terms' magnitudes (returned in that
+    order after the sums)."""
     lib = _load("liboracle.so")
     a = _BoxArgs(box)
     ss = np.ascontiguousarray(sink_slab, np.int32)
@@ -87,11 +89,13 @@ def direct_f64(box, R, kind, eps, b2, sink_slab, sink_index, sink_cell, with_sof
     sc = np.ascontiguousarray(sink_cell, np.int32)
     out = np.zeros((ss.size, 4))
     soft = np.zeros(ss.size) if with_soft else None
+    mag = np.zeros(ss.size) if with_abs else None
     lib.or_direct_f64(C.c_int(box.cpd), a.pos, _p(a.nslab), a.start, a.count, C.c_int(R),
                       C.c_int(kind_of(kind)), C.c_double(float(eps)), C.c_double(float(b2)),
                       C.c_int64(ss.size), _p(ss), _p(si), _p(sc), _p(out),
-                      _p(soft) if with_soft else None)
-    return (out, soft) if with_soft else out
+                      _p(soft) if with_soft else None, _p(mag) if with_abs else None)
+    extra = [v for v, on in ((soft, with_soft), (mag, with_abs)) if on]
+    return tuple([out] + extra) if extra else out
 
 
 def slab_sinks(box, x):

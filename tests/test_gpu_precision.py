@@ -35,21 +35,31 @@ def test_precision_study_uniform64():
             st = [s for x in range(box.cpd) for s in drv.finalize(x)]
         res[path] = np.concatenate([accs[x] for x in range(box.cpd)])
         rate[path] = sum(s["direct_total"] for s in st) / max(sum(s["direct_kernel_s"] for s in st), 1e-30)
-    refs, softs, cpu = [], [], []
+    refs, softs, mags, cpu = [], [], [], []
     for x in range(box.cpd):
         sl, ix, ce = po.slab_sinks(box, x)
-        r, s = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce, with_soft=True)
+        r, s, m = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce, with_soft=True, with_abs=True)
         refs.append(r)
         softs.append(s)
+        mags.append(m)
         cpu.append(po.execute_slab_cpu(box, R, "single_spline", eps, x)[0])
-    ref, soft, cpu = np.concatenate(refs), np.concatenate(softs), np.concatenate(cpu)
+    ref, soft, mag, cpu = (np.concatenate(v) for v in (refs, softs, mags, cpu))
     keep = not_soft_dominated(ref, soft)
+    # Condition number of each sink's force sum: sum of term magnitudes / |a|.
+    # FP32 terms carry relative errors of ~1e-7, so a sink's attainable
+    # relative error grows with it.
+    cond = mag / np.maximum(np.linalg.norm(ref[:, :3], axis=1), 1e-300)
+    well = keep & (cond <= 100.0)
     report = {"workload": box.name, "sinks": int(ref.shape[0]),
-              "softening_dominated_sinks": int((~keep).sum())}
+              "softening_dominated_sinks": int((~keep).sum()),
+              "condition_number": {"median": float(np.median(cond)), "p99": float(np.percentile(cond, 99)),
+                                   "max": float(cond.max()), "sinks_above_100": int((cond > 100).sum())}}
     for name, acc in (("plain", res["plain"]), ("batched", res["batched"]), ("cpu_directs", cpu)):
         rel = rel_err(acc, ref)
         report[name] = {"all": summarize(rel, tail=True),
-                        "outside_softening_dominated": summarize(rel, keep, tail=True)}
+                        "outside_softening_dominated": summarize(rel, keep, tail=True),
+                        "condition_number_le_100": summarize(rel, well, tail=True),
+                        "max_error_over_condition_number": float((rel[keep] / cond[keep]).max())}
         if name != "cpu_directs":
             report[name]["kernel_interactions_per_s"] = rate[name]
             report[name]["vs_cpu_directs"] = summarize(rel_err(acc, cpu))
@@ -60,14 +70,20 @@ def test_precision_study_uniform64():
             json.dump(report, f, indent=1)
     b, p = report["batched"]["outside_softening_dominated"], report["plain"]["outside_softening_dominated"]
     c = report["cpu_directs"]["outside_softening_dominated"]
-    # rms within 1e-6 of the FP64 oracle; the max is set by a few sinks whose
-    # net force nearly cancels (FP32 tile sums of ~6,700 terms): bounded here,
-    # and better than both the reference-structured FP32 path and the
-    # reference CPU directs on the same inputs.
-    assert b["rms"] <= 1e-6 and b["max"] <= 5e-5, b
+    # rms within 1e-6 of the FP64 oracle (measured 2.2e-7).  The max is set by
+    # a few sinks whose net force nearly cancels (condition number 100-1000):
+    # with FP32 pair terms their relative error is ~1e-7 x the condition
+    # number, which the batch and tile sums do not change.  Bounded here, and
+    # better than both the reference-structured FP32 path and the reference
+    # CPU directs on the same inputs; on sinks with condition number <= 100
+    # the max is within 1e-5.
+    assert b["rms"] <= 4e-7 and b["max"] <= 3e-5, b
     assert b["rms"] < p["rms"] and b["rms"] < c["rms"]
     assert b["max"] < p["max"] and b["max"] < c["max"]
-    assert b["above_1e5"] <= 1e-4 * b["n"]
+    assert b["above_1e5"] <= 2e-5 * b["n"]
+    bw = report["batched"]["condition_number_le_100"]
+    assert bw["max"] <= 1e-5 and bw["n"] >= 0.99 * b["n"], bw
+    assert report["batched"]["max_error_over_condition_number"] <= 1e-6
     assert report["batched"]["vs_cpu_directs"]["rms"] <= 1e-6
     # the FOF density w against FP64 (self term included)
     np.testing.assert_allclose(res["batched"][:, 3], ref[:, 3], rtol=2e-6, atol=1e-6 * float(b2))

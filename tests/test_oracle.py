@@ -155,7 +155,9 @@ def test_fp64_two_particles(p, q):
         s = box.slab(x)
         if s.n:
             sl, ix, ce = po.slab_sinks(box, x)
-            res = po.direct_f64(box, R, "single_spline", eps, 0.0, sl, ix, ce)
+            res, mag = po.direct_f64(box, R, "single_spline", eps, 0.0, sl, ix, ce, with_abs=True)
+            # one term per sink: the sum of term magnitudes is |a| (condition number 1)
+            np.testing.assert_allclose(mag, np.linalg.norm(res[:, :3], axis=1), rtol=1e-12)
             glob = (np.stack([np.full(s.n, x), ce // cpd, ce % cpd], 1) - (cpd - 1) // 2) / cpd + s.xyz
             for g, a in zip(glob, res):
                 out[tuple(np.round(g, 6))] = a
@@ -176,9 +178,14 @@ def test_fp64_lattice_has_zero_force():
     box = synth.box_from_global(g + 1e-12, cpd)
     e = 1e-3
     sl, ix, ce = po.slab_sinks(box, 2)
-    a = po.direct_f64(box, R, "single_spline", 1 / e ** 2, 0.0, sl, ix, ce)
+    a, mag = po.direct_f64(box, R, "single_spline", 1 / e ** 2, 0.0, sl, ix, ce, with_abs=True)
     typical = e ** 3 / (1.0 / cpd) ** 2
     assert np.abs(a[:, :3]).max() < 1e-9 * typical
+    # The sum of the 124 Newtonian term magnitudes e^3/r^2, r = |(i,j,k)|/cpd.
+    ijk = np.stack(np.meshgrid(*[np.arange(-R, R + 1)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    r2 = (ijk ** 2).sum(1)
+    expect = e ** 3 * cpd ** 2 * (1.0 / r2[r2 > 0]).sum()
+    np.testing.assert_allclose(mag, expect, rtol=1e-9)
 
 
 def test_fp64_momentum_conservation():
