@@ -61,9 +61,17 @@ __device__ __forceinline__ int cell_of(int i, const int *pre, int W) {
 
 // One wave per padded 64-sink block (the reference's SinkBlockParentPencil
 // names its pencil), so a heavy pencil spreads over many waves.
+// Position of particle idx of a slab of n particles: PosXYZ (X|Y|Z) or, with
+// aos, posstruct (x,y,z interleaved).
+__device__ __forceinline__ float3 slab_xyz(const float *P, uint64_t n, uint64_t idx, int aos) {
+    if (aos) return make_float3(P[3 * idx], P[3 * idx + 1], P[3 * idx + 2]);
+    return make_float3(P[idx], P[n + idx], P[2 * n + idx]);
+}
+
 template <bool F64>
 __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *__restrict__ slab, uint64_t n,
-                                                      float4 *__restrict__ out, float cellsize, double invcpd) {
+                                                      int aos, float4 *__restrict__ out, float cellsize,
+                                                      double invcpd) {
     const int b = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
     if (b >= pl.nsink_blocks) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -79,9 +87,10 @@ __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *_
     if (i < count) {
         const int c = cell_of(i, pre, W);
         const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
-        v.x = slab[idx];
-        v.y = slab[n + idx];
-        float z = slab[2 * n + idx];
+        const float3 q = slab_xyz(slab, n, idx, aos);
+        v.x = q.x;
+        v.y = q.y;
+        float z = q.z;
         const int dc = c - pl.R;
         if (F64) {
             z = (float)((double)z + (double)dc * invcpd);
@@ -133,9 +142,8 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
     if (real) {
         const int c = cell_of(i, pre, W);
         const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
-        const float *P = slab_ptr(sl, c);
-        const uint64_t m = slab_n(sl, c);
-        float x = P[idx];
+        const float3 q = slab_xyz(slab_ptr(sl, c), slab_n(sl, c), idx, sl.aos);
+        float x = q.x;
         const int dc = c - pl.R;
         if (F64) {
             x = (float)((double)x + (double)dc * invcpd);
@@ -143,7 +151,7 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
             const float dx = (float)dc * cellsize;
             if (dx != 0.f) x = x + dx;
         }
-        v = make_float4(x, P[m + idx], P[2 * m + idx], 0.f);
+        v = make_float4(x, q.y, q.z, 0.f);
     }
     out[(int64_t)b * kWave + lane] = v;
     const float inf = __builtin_huge_valf();
@@ -258,8 +266,23 @@ __device__ __forceinline__ void partial_slots(const DevPlan &pl, int j, int z, i
 // in increasing k, the first one setting the value and the rest adding.  For
 // cell z that is c = min(z, W-1) .. 0, then W-1 .. z+1.  One wave per chunk
 // of 64 particles of a sink cell.
+// RescaleAndCoAddAcceleration (kick.cpp:96-105) on one sink's float4, with
+// the FLOAT3p1 rules (float3p1.cpp:41-56): x, y, z = (a * inv_eps3 + far) *
+// rescale in float, each operation rounded on its own (no contraction), and
+// w -= DensityKernelRad2 in double.
+__device__ __forceinline__ float4 apply_kick(float4 a, const KickArgs &k, int64_t row) {
+#pragma clang fp contract(off)
+    if (!k.on) return a;
+    const float *f = k.far + 3 * (row - k.far_base);
+    a.x = (a.x * k.inv_eps3 + f[0]) * k.rescale;
+    a.y = (a.y * k.inv_eps3 + f[1]) * k.rescale;
+    a.z = (a.z * k.inv_eps3 + f[2]) * k.rescale;
+    a.w = (float)((double)a.w - k.density_rad2);
+    return a;
+}
+
 __global__ __launch_bounds__(256) void k_coadd_plain(DevPlan pl, const float4 *__restrict__ part,
-                                                     float4 *__restrict__ acc) {
+                                                     float4 *__restrict__ acc, KickArgs kick) {
     const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
     if (q >= pl.n_coadd_chunks) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -283,13 +306,13 @@ __global__ __launch_bounds__(256) void k_coadd_plain(DevPlan pl, const float4 *_
         a.z += p.z;
         a.w += p.w;
     }
-    acc[cstart + i] = a;
+    acc[cstart + i] = apply_kick(a, kick, (int64_t)cstart + i);
 }
 
 // FP64 co-add: sum of the W double partials, scaled back to the reference's
 // units (out_scale applies to a, not to w).  One wave per 64-particle chunk.
 __global__ __launch_bounds__(256) void k_coadd_f64(DevPlan pl, const double *__restrict__ part, double out_scale,
-                                                   float4 *__restrict__ acc) {
+                                                   float4 *__restrict__ acc, KickArgs kick) {
     const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
     if (q >= pl.n_coadd_chunks) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -310,7 +333,8 @@ __global__ __launch_bounds__(256) void k_coadd_f64(DevPlan pl, const double *__r
         a2 += p[2];
         a3 += p[3];
     }
-    acc[cstart + i] = make_float4((float)(a0 * out_scale), (float)(a1 * out_scale), (float)(a2 * out_scale), (float)a3);
+    const float4 a = make_float4((float)(a0 * out_scale), (float)(a1 * out_scale), (float)(a2 * out_scale), (float)a3);
+    acc[cstart + i] = apply_kick(a, kick, (int64_t)cstart + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -741,10 +765,10 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void
     if (pl.nsink_blocks > 0) {
         const dim3 grid((pl.nsink_blocks + 3) / 4);
         if (fp64_offsets)
-            hipLaunchKernelGGL(k_gather_sinks<true>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R],
+            hipLaunchKernelGGL(k_gather_sinks<true>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R], sl.aos,
                                static_cast<float4 *>(sinks4), cellsize, invcpd);
         else
-            hipLaunchKernelGGL(k_gather_sinks<false>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R],
+            hipLaunchKernelGGL(k_gather_sinks<false>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R], sl.aos,
                                static_cast<float4 *>(sinks4), cellsize, invcpd);
     }
     if (pl.nsrc_blocks > 0) {
@@ -774,19 +798,20 @@ int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const vo
     }
 }
 
-int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, void *stream) {
+int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, const KickArgs &kick, void *stream) {
     if (pl.n_coadd_chunks == 0) return 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_coadd_plain, dim3((pl.n_coadd_chunks + 3) / 4), dim3(256), 0, st, pl,
-                       static_cast<const float4 *>(partial4), reinterpret_cast<float4 *>(acc4));
+                       static_cast<const float4 *>(partial4), reinterpret_cast<float4 *>(acc4), kick);
     return (int)hipGetLastError();
 }
 
-int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4, void *stream) {
+int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
+                     const KickArgs &kick, void *stream) {
     if (pl.n_coadd_chunks == 0) return 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_coadd_f64, dim3((pl.n_coadd_chunks + 3) / 4), dim3(256), 0, st, pl,
-                       static_cast<const double *>(partial_d4), out_scale, reinterpret_cast<float4 *>(acc4));
+                       static_cast<const double *>(partial_d4), out_scale, reinterpret_cast<float4 *>(acc4), kick);
     return (int)hipGetLastError();
 }
 

@@ -62,8 +62,20 @@ struct DevPlan {
 
 // The slabs of one SIC on the device.
 struct DevSlabs {
-    const float *pos[NF_MAX_WIDTH];  // PosXYZ slab of SlabId-R+c
+    const float *pos[NF_MAX_WIDTH];  // position slab of SlabId-R+c
     uint64_t n[NF_MAX_WIDTH];
+    int32_t aos;                     // 1: posstruct (x,y,z) per particle; 0: PosXYZ X|Y|Z
+};
+
+// The consumer's RescaleAndCoAddAcceleration, applied by the co-add kernels
+// to each sink's float4 (on == 0: off).  far[3 (row - far_base) + k] is the
+// far-field acceleration of slab row `row`.
+struct KickArgs {
+    const float *far;
+    int64_t far_base;
+    float rescale, inv_eps3;
+    double density_rad2;
+    int32_t on;
 };
 
 // Constants of the pair kernel, derived on the host from (softening, eps, b2).
@@ -94,7 +106,7 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void
                   void *sources4, void *stream);
 int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const void *sinks4,
                  const void *sources4, void *partial4, void *stream);
-int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, void *stream);
+int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, const KickArgs &kick, void *stream);
 // One persistent launch over the work items of a group of SICs.
 int launch_batched(const SicView *views, int cpd, int softening, const PairConst &pc, const TileShape &ts,
                    const WorkItem *items, int32_t nitems, int32_t *counter, void *stream);
@@ -103,7 +115,7 @@ inline void *box_after(void *sources4, int32_t nsrc_blocks) {
     return static_cast<char *>(sources4) + (size_t)nsrc_blocks * NF_BLOCK_SIZE * 16;
 }
 int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
-                     void *stream);
+                     const KickArgs &kick, void *stream);
 int device_cu_count();
 // The occupancy target (waves per SIMD) the batched kernel is built with for
 // a pair form and tile; the third template argument of k_batched.

@@ -149,7 +149,7 @@ struct Worker {
     hipGraphExec_t gexec = nullptr;  // the last group's executable graph (use_graphs)
     uint64_t graphs_instantiated = 0, graphs_updated = 0;
     DevBuf plan, work, slabs;
-    PinBuf pin_plan, pin_acc;
+    PinBuf pin_plan, pin_acc, pin_far;
     std::thread th;
     bool ready = false;
 };
@@ -256,7 +256,7 @@ PlanLayout plan_layout(const nf_sic &s, size_t base) {
 
 // Device working set of one SIC: sink stream, source stream + boxes, partials.
 struct WorkLayout {
-    size_t o_sinks, o_src, o_part, total;
+    size_t o_sinks, o_src, o_part, o_far, total;
 };
 
 WorkLayout work_layout(const nf_sic &s, int path, size_t base) {
@@ -266,9 +266,21 @@ WorkLayout work_layout(const nf_sic &s, int path, size_t base) {
     L.o_sinks = base;
     L.o_src = align_up(L.o_sinks + 16 * psink, 256);
     L.o_part = align_up(L.o_src + 16 * psrc + 32 * (size_t)s.NSourceBlocks, 256);
-    const size_t end = align_up(L.o_part + psink * (path == NF_PATH_BATCHED ? 32 : 16), 256);
+    size_t end = align_up(L.o_part + psink * (path == NF_PATH_BATCHED ? 32 : 16), 256);
+    L.o_far = end;  // host mode with a fused kick: the sink rows' far-field accelerations
+    if (s.kick && s.mem == NF_MEM_HOST) end = align_up(end + 12 * (size_t)s.Nslab[s.nfradius], 256);
     L.total = end - base;
     return L;
+}
+
+// Rows [r0, r1) of the sink slab a SIC covers.  Cells are y-major, so rows
+// j_low..j_high-1 are contiguous: from cell (j_low, 0) -- the first cell of
+// pencil 0 -- to the end of cell (j_high-1, cpd-1) -- the first cell of the
+// last pencil.
+void sink_rows(const nf_sic &s, uint64_t *r0, uint64_t *r1) {
+    *r0 = s.SinkPlan[0].cell[0].start;
+    const nf_sink_plan &last = s.SinkPlan[(size_t)(s.j_width - 1) * s.Nk + (s.Nk - 1)];
+    *r1 = (uint64_t)last.cell[0].start + (uint64_t)last.cell[0].N;
 }
 
 // Device bytes a SIC adds to a group (plan + work + host-mode slabs).
@@ -328,6 +340,10 @@ int validate(const nf_sic &s) {
         (s.NSinkBlocks > 0 && !s.SinkBlockParentPencil))
         return fail(NF_ERR_INVALID, "missing plan array");
     if (!s.SinkPosSlab || !s.SinkAccSlab) return fail(NF_ERR_INVALID, "missing slab pointer");
+    if (s.pos_layout != NF_POS_XYZ && s.pos_layout != NF_POS_AOS) return fail(NF_ERR_INVALID, "bad pos_layout");
+    if (s.kick && (!s.kick->FarAccSlab || !std::isfinite(s.kick->rescale) || !std::isfinite(s.kick->inv_eps3) ||
+                   !std::isfinite(s.kick->density_rad2)))
+        return fail(NF_ERR_INVALID, "kick fusion needs a far-field slab and finite constants");
     for (int c = 0; c < s.nfwidth; c++)
         if (!s.SourcePosSlab[c]) return fail(NF_ERR_INVALID, "missing source slab pointer");
     if (s.mem != NF_MEM_HOST && s.mem != NF_MEM_DEVICE) return fail(NF_ERR_INVALID, "bad mem kind");
@@ -707,6 +723,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     char *db = static_cast<char *>(w.slabs.p);
     std::vector<DevPlan> plans(n);
     std::vector<DevSlabs> slabs(n);
+    std::vector<KickArgs> kicks(n);
     std::vector<float *> accd(n);
     SicView *views = reinterpret_cast<SicView *>(hp + o_views);
     for (int i = 0; i < n; i++) {
@@ -776,7 +793,20 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             }
         }
         if (s.mem == NF_MEM_DEVICE) sl.pos[s.nfradius] = s.SinkPosSlab;
+        sl.aos = s.pos_layout == NF_POS_AOS ? 1 : 0;
         accd[i] = s.mem == NF_MEM_HOST ? reinterpret_cast<float *>(db + acc_off[i]) : s.SinkAccSlab;
+        KickArgs &ka = kicks[i];
+        std::memset(&ka, 0, sizeof(ka));
+        if (s.kick) {
+            uint64_t r0, r1;
+            sink_rows(s, &r0, &r1);
+            ka.on = 1;
+            ka.rescale = (float)s.kick->rescale;
+            ka.inv_eps3 = (float)s.kick->inv_eps3;
+            ka.density_rad2 = s.kick->density_rad2;
+            ka.far = s.mem == NF_MEM_HOST ? reinterpret_cast<const float *>(wb + WL[i].o_far) : s.kick->FarAccSlab;
+            ka.far_base = s.mem == NF_MEM_HOST ? (int64_t)r0 : 0;
+        }
 
         SicView &v = views[i];
         std::memset(&v, 0, sizeof(v));
@@ -799,13 +829,42 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     // the group before (this worker's previous group has completed, so its
     // plan buffer is free).
     NF_HIP(hipMemcpyAsync(w.plan.p, hp, plan_total, hipMemcpyHostToDevice, rt.cache.up));
+    // Host mode with a fused kick: the far-field accelerations of the sink
+    // rows go up beside the plan (pinned ones directly, else through staging).
+    size_t far_bytes = 0;
+    for (int i = 0; i < n; i++) {
+        const nf_sic &s = g[i]->sic;
+        if (!s.kick || s.mem != NF_MEM_HOST) continue;
+        uint64_t r0, r1;
+        sink_rows(s, &r0, &r1);
+        if (r1 > r0) far_bytes += 12 * (size_t)(r1 - r0);
+    }
+    if (far_bytes) {
+        if ((rc = w.pin_far.ensure(far_bytes))) return rc;
+        size_t off = 0;
+        for (int i = 0; i < n; i++) {
+            const nf_sic &s = g[i]->sic;
+            if (!s.kick || s.mem != NF_MEM_HOST) continue;
+            uint64_t r0, r1;
+            sink_rows(s, &r0, &r1);
+            if (r1 <= r0) continue;
+            const size_t nb = 12 * (size_t)(r1 - r0);
+            const float *src = s.kick->FarAccSlab + 3 * r0;
+            if (!host_is_pinned(src)) {
+                par_memcpy(static_cast<char *>(w.pin_far.p) + off, src, nb);
+                src = reinterpret_cast<const float *>(static_cast<char *>(w.pin_far.p) + off);
+            }
+            NF_HIP(hipMemcpyAsync(wb + WL[i].o_far, src, nb, hipMemcpyHostToDevice, rt.cache.up));
+            off += nb;
+        }
+    }
     NF_HIP(hipEventRecord(w.plan_ready, rt.cache.up));
 
     // ---- Enqueue the group as one uninterrupted sequence.
     hipStream_t stream = rt.shared ? rt.shared : w.stream;
     std::unique_lock<std::mutex> enqueue_lock(rt.enqueue, std::defer_lock);
     if (rt.shared) enqueue_lock.lock();
-    uint64_t to_dev = plan_total, from_dev = 0;
+    uint64_t to_dev = plan_total + far_bytes, from_dev = 0;
     // With graphs, the sequence below is captured and launched as one graph.
     const bool use_graph = rt.params.use_graphs && !trial;
     struct CaptureGuard {  // ends an open capture on every early return
@@ -867,8 +926,9 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     if (e) return fail(NF_ERR_HIP, std::string("direct kernel launch: ") + hipGetErrorString((hipError_t)e));
     if (!use_graph) NF_HIP(hipEventRecord(w.ev[3], stream));
     for (int i = 0; i < n && !e; i++) {
-        if (path == NF_PATH_BATCHED) e = launch_coadd_f64(plans[i], wb + WL[i].o_part, pc.out_scale, accd[i], stream);
-        else e = launch_coadd_plain(plans[i], wb + WL[i].o_part, accd[i], stream);
+        if (path == NF_PATH_BATCHED)
+            e = launch_coadd_f64(plans[i], wb + WL[i].o_part, pc.out_scale, accd[i], kicks[i], stream);
+        else e = launch_coadd_plain(plans[i], wb + WL[i].o_part, accd[i], kicks[i], stream);
     }
     if (e) return fail(NF_ERR_HIP, std::string("co-add launch: ") + hipGetErrorString((hipError_t)e));
     if (use_graph) {
@@ -927,12 +987,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         const auto tc = clk::now();
         for (int i = 0; i < n; i++) {
             const nf_sic &s = g[i]->sic;
-            // Rows [j_low, j_high) are contiguous (cells are y-major): from cell
-            // (j_low, 0) -- pencil 0's first cell -- to the end of cell
-            // (j_high-1, cpd-1) -- the first cell of the last pencil.
-            const uint64_t r0 = s.SinkPlan[0].cell[0].start;
-            const nf_sink_plan &last = s.SinkPlan[(size_t)(s.j_width - 1) * s.Nk + (s.Nk - 1)];
-            const uint64_t r1 = (uint64_t)last.cell[0].start + (uint64_t)last.cell[0].N;
+            uint64_t r0, r1;
+            sink_rows(s, &r0, &r1);
             if (r1 <= r0) continue;
             // The copy-back runs on the download stream, after the group's
             // kernels, while the next group's kernels use the compute stream:
@@ -1060,6 +1116,7 @@ void destroy_worker(Worker &w) {
     w.slabs.release();
     w.pin_plan.release();
     w.pin_acc.release();
+    w.pin_far.release();
     if (w.gexec) (void)hipGraphExecDestroy(w.gexec);
     w.gexec = nullptr;
     for (auto &e : w.ev)

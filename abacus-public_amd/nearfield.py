@@ -61,7 +61,16 @@ class nf_sic(C.Structure):
         ("SinkBlockParentPencil", _P), ("SinkSourceInteractionList", _P), ("SinkSourceYOffset", _P),
         ("DirectTotal", C.c_uint64), ("PaddedDirectTotal", C.c_uint64), ("SinkTotal", C.c_uint64),
         ("SourceTotal", C.c_uint64), ("PaddedSinkTotal", C.c_uint64), ("PaddedSourceTotal", C.c_uint64),
+        ("pos_layout", C.c_int32), ("reserved", C.c_int32), ("kick", _P),
     ]
+
+
+class nf_kick(C.Structure):
+    _fields_ = [("FarAccSlab", _P), ("rescale", C.c_double), ("inv_eps3", C.c_double),
+                ("density_rad2", C.c_double)]
+
+
+NF_POS_XYZ, NF_POS_AOS = 0, 1
 
 
 class nf_task_stats(C.Structure):
@@ -221,7 +230,7 @@ class SetInteractionCollection:
     """A planned SIC: the ``nf_sic`` descriptor and the arena behind it."""
 
     def __init__(self, box, slab, j_low, j_high, nfradius, eps, b2, softening,
-                 pos_ptrs=None, acc_ptr=None, mem=NF_MEM_HOST):
+                 pos_ptrs=None, acc_ptr=None, mem=NF_MEM_HOST, pos_layout=NF_POS_XYZ, kick=None):
         cpd = box.cpd
         W = 2 * nfradius + 1
         slabs = [box.slab(slab - nfradius + c) for c in range(W)]
@@ -241,6 +250,12 @@ class SetInteractionCollection:
         self.desc.mem = mem
         if acc_ptr is not None:
             self.desc.SinkAccSlab = acc_ptr
+        self.desc.pos_layout = pos_layout
+        self.kick = None
+        if kick is not None:
+            # nf_kick(FarAccSlab=pointer, rescale=, inv_eps3=, density_rad2=); kept alive here
+            self.kick = kick
+            self.desc.kick = C.addressof(kick)
 
     # Views of the planned arrays (for tests and statistics).
     def arrays(self):
@@ -330,11 +345,13 @@ class NearFieldDriver:
         self.max_sink_blocks, self.max_source_blocks = msb.value, mrb.value
         self.slab_tasks = {}
 
-    def plan(self, box, slab, j_low=0, j_high=None, pos_ptrs=None, acc_ptr=None, mem=NF_MEM_HOST):
+    def plan(self, box, slab, j_low=0, j_high=None, pos_ptrs=None, acc_ptr=None, mem=NF_MEM_HOST,
+             pos_layout=NF_POS_XYZ, kick=None):
         if j_high is None:
             j_high = self.cpd
         return SetInteractionCollection(box, slab, j_low, j_high, self.R, self.eps, self.b2, self.softening,
-                                        pos_ptrs=pos_ptrs, acc_ptr=acc_ptr, mem=mem)
+                                        pos_ptrs=pos_ptrs, acc_ptr=acc_ptr, mem=mem, pos_layout=pos_layout,
+                                        kick=kick)
 
     def submit(self, sic, blocking=False):
         h = C.c_void_p()

@@ -59,6 +59,30 @@ typedef enum nf_path { NF_PATH_PLAIN = 0, NF_PATH_BATCHED = 1 } nf_path;
 /* Where the slab pointers of an nf_sic live. */
 typedef enum nf_mem { NF_MEM_HOST = 0, NF_MEM_DEVICE = 1 } nf_mem;
 
+/* Layout of the position slabs a SIC names.  NF_POS_XYZ is the PosXYZ slab
+ * (X[n] | Y[n] | Z[n], List3, cellparticles.cpp:218-228).  NF_POS_AOS is the
+ * PosSlab itself (posstruct x,y,z per particle, the same indexing): the
+ * gathers read it directly, so the TransposePos pass (pipeline.cpp:107-124)
+ * is not needed.  No counterpart in the reference's GPU module. */
+typedef enum nf_pos_layout { NF_POS_XYZ = 0, NF_POS_AOS = 1 } nf_pos_layout;
+
+/* Fusion of the consumer RescaleAndCoAddAcceleration (kick.cpp:74-109) into
+ * the co-add; no counterpart in the reference's GPU module.  With it the SIC's
+ * sink rows of SinkAccSlab receive the final acceleration,
+ *   acc.xyz = (near.xyz * inv_eps3 + FarAcc.xyz) * rescale,
+ *   acc.w   = near.w - density_rad2
+ * (FLOAT3p1 arithmetic: w takes no part in the scaling, float3p1.cpp:41-56),
+ * instead of the near-field sum: the same float operations, in the same
+ * order, as running the module and then RescaleAndCoAddAcceleration. */
+typedef struct nf_kick {
+    const float *FarAccSlab; /* acc3struct (x, y, z) per particle of slab SlabId, in the SIC's nf_mem */
+    double rescale;          /* -3 (Omega_M - Omega_Smooth) / (8 pi np), negated when making glass;
+                                rounded to float, as the reference's FLOAT */
+    double inv_eps3;         /* 1 / SofteningLengthInternal^3 for the single spline, else 1;
+                                rounded to float */
+    double density_rad2;     /* WriteState.DensityKernelRad2 (double, as the reference) */
+} nf_kick;
+
 /* CellPencilPlan (SetInteractionCollection.hh:76-87): start index of a cell
  * in its PosXYZ slab and its particle count.  Same layout as the reference. */
 typedef struct nf_cell_plan {
@@ -121,6 +145,11 @@ typedef struct nf_sic {
 
     uint64_t DirectTotal, PaddedDirectTotal;
     uint64_t SinkTotal, SourceTotal, PaddedSinkTotal, PaddedSourceTotal;
+
+    /* Extensions (zero for the reference's behaviour). */
+    int32_t pos_layout;      /* nf_pos_layout of SinkPosSlab and SourcePosSlab */
+    int32_t reserved;
+    const nf_kick *kick;     /* non-NULL: write the final acceleration (see nf_kick) */
 } nf_sic;
 
 /* Per-task timers and counters, the analogue of the SIC's STimers and byte

@@ -208,3 +208,22 @@ def pencil_execute(plan, sinks4, sources4, R, kind, eps, b2):
                           _p(idm), _p(ss), _p(sc), _p(il), _p(yo), _p(k4), _p(s4), C.c_float(eps), C.c_float(b2),
                           _p(out))
     return out
+
+
+# ---- consumer of the near field ---------------------------------------------
+
+def rescale_coadd(nacc, facc, rescale, inv_eps3, density_rad2):
+    """RescaleAndCoAddAcceleration (kick.cpp:74-109) for the single-spline,
+    FOF-density build: nacc is the float4 near-field slab (FLOAT3p1), facc the
+    float3 far-field slab (acc3struct).  FLOAT3p1 rules (float3p1.cpp:41-56):
+    scaling by a float and adding a float3 leave w alone.  rescale and
+    inv_eps3 are FLOAT (rounded to float32); each operation is rounded on its
+    own; w -= DensityKernelRad2 is evaluated in double (a double member,
+    statestructure.cpp:102) and rounded back."""
+    n = np.asarray(nacc, np.float32).reshape(-1, 4)
+    f = np.asarray(facc, np.float32).reshape(-1, 3)
+    r, ie = np.float32(rescale), np.float32(inv_eps3)
+    out = np.empty_like(n)
+    out[:, :3] = ((n[:, :3] * ie) + f) * r
+    out[:, 3] = (n[:, 3].astype(np.float64) - float(density_rad2)).astype(np.float32)
+    return out

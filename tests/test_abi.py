@@ -39,7 +39,8 @@ def test_struct_layout_matches_c(tmp_path):
            ' sizeof(nf_cellinfo), sizeof(nf_task_stats));']
     for f in fields:
         src.append(f'printf("%zu\\n", offsetof(nf_sic, {f}));')
-    src.append('printf("%zu\\n", sizeof(nf_setup_params)); return 0; }')
+    src.append('printf("%zu %zu %zu\\n", offsetof(nf_kick, density_rad2), sizeof(nf_kick), sizeof(nf_setup_params));'
+               ' return 0; }')
     c = tmp_path / "probe.c"
     c.write_text("\n".join(src))
     exe = tmp_path / "probe"
@@ -52,7 +53,8 @@ def test_struct_layout_matches_c(tmp_path):
     assert sizes[1] == 44 and sizes[2] == 40
     offs = list(map(int, out[5:5 + len(fields)]))
     assert offs == [getattr(nfm.nf_sic, f).offset for f in fields]
-    assert int(out[-1]) == C.sizeof(nfm.nf_setup_params)
+    assert list(map(int, out[-3:])) == [nfm.nf_kick.density_rad2.offset, C.sizeof(nfm.nf_kick),
+                                        C.sizeof(nfm.nf_setup_params)]
 
 
 def test_plan_errors_are_codes():

@@ -249,3 +249,22 @@ def test_pencil_restatement_matches_cell_pair():
     acc_c, _ = po.execute_slab_cpu(box, R, "single_spline", eps, x)
     st = summarize(rel_err(acc_p, acc_c))
     assert st["rms"] < 3e-6 and st["max"] < 1e-4, st
+
+
+def test_rescale_coadd_known_answers():
+    """RescaleAndCoAddAcceleration restated: xyz = (near*inv_eps3 + far)*rescale
+    with float rounding per operation, w untouched by the scaling and reduced
+    by the density radius^2 in double.  Values chosen so each step is exact or
+    checkable by hand."""
+    nacc = np.array([[1.0, -2.0, 0.5, 3.0], [0.0, 0.0, 0.0, 0.25]], np.float32)
+    facc = np.array([[0.5, 0.5, -0.5], [1.0, 2.0, 4.0]], np.float32)
+    out = po.rescale_coadd(nacc, facc, rescale=-2.0, inv_eps3=4.0, density_rad2=0.25)
+    np.testing.assert_array_equal(out, np.array([[-9.0, 15.0, -3.0, 2.75], [-2.0, -4.0, -8.0, 0.0]], np.float32))
+    # float rounding of each step: 1 + 2^-24 is not representable, the sum rounds to 1
+    o = po.rescale_coadd(np.array([[1.0, 0, 0, 0]], np.float32), np.array([[2.0 ** -25, 0, 0]], np.float32),
+                         1.0, 1.0, 0.0)
+    assert o[0, 0] == np.float32(1.0)
+    # rescale and inv_eps3 are rounded to float32 first (FLOAT), w in double
+    o = po.rescale_coadd(np.array([[1.0, 0, 0, 1.0]], np.float32), np.zeros((1, 3), np.float32),
+                         1.0 / 3.0, 1.0, 1e-9)
+    assert o[0, 0] == np.float32(1.0 / 3.0) and o[0, 3] == np.float32(1.0 - 1e-9)
