@@ -64,6 +64,10 @@ def main():
     ap.add_argument("--stream-slabs", type=int, default=40, help="slab-stream workloads: sink slabs per pass")
     ap.add_argument("--pinned", action="store_true",
                     help="slab streams: host slabs in pinned memory (direct DMA) instead of pageable")
+    ap.add_argument("--aos", action="store_true",
+                    help="slab streams: hand the gathers posstruct (x,y,z per particle) slabs, skipping TransposePos")
+    ap.add_argument("--fuse-kick", action="store_true",
+                    help="slab streams: fuse RescaleAndCoAddAcceleration (far field in, final acceleration out)")
     ap.add_argument("--graphs", type=int, default=None,
                     help="capture each group's kernels in a HIP graph (default: on for slab streams)")
     ap.add_argument("--paths", default="plain,batched")
@@ -388,11 +392,26 @@ def run_stream(args):
         box = synth.Box(cpd, [synth.Slab(host_array(s.pos.shape, s.pos), s.count) for s in box.slabs], box.name)
         box.N = n1d ** 3
     acc = {x: host_array((box.slab(x).n, 4)) for x in sinks}
+    # posstruct copies of the slabs (what the reference's PosSlab holds before TransposePos)
+    aos = {x: host_array((box.slab(x).n * 3,), box.slab(x).xyz.reshape(-1)) for x in range(-R, len(sinks) + R)} \
+        if args.aos else None
+    kicks = {}
+    if args.fuse_kick:
+        rng = np.random.default_rng(7)
+        _, e_int = synth.eps_param("single_spline", soft_len)
+        for x in sinks:
+            far = host_array((box.slab(x).n * 3,), rng.standard_normal(box.slab(x).n * 3).astype(np.float32))
+            kicks[x] = nfm.nf_kick(FarAccSlab=far.ctypes.data, rescale=-3.0 * 0.3 / (8.0 * np.pi * box.N),
+                                   inv_eps3=1.0 / e_int ** 3, density_rad2=float(b2))
+            keep.append(far)
     with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path="batched", buffers_per_device=args.bpd,
                              max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
                              gpu_streams=args.gpu_streams, max_group=args.max_group, use_graphs=graphs) as drv:
         tp = time.time()
-        sics = [drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST) for x in sinks]
+        sics = [drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST,
+                         pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data for c in range(2 * R + 1)],
+                         pos_layout=nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ, kick=kicks.get(x))
+                for x in sinks]
         plan_s = time.time() - tp
         direct_total = sum(int(s.desc.DirectTotal) for s in sics)
 
@@ -441,7 +460,8 @@ def run_stream(args):
         "config": {"workload": desc, "cpd": cpd, "N_nominal": n1d ** 3, "sink_slabs": len(sinks),
                    "particles_in_window": int(sum(box.slab(x).n for x in range(-R, len(sinks) + R))),
                    "interactions_per_step": direct_total, "path": "batched", "graphs": graphs,
-                   "softening": "single_spline", "mem": "host pinned" if args.pinned else "host pageable"},
+                   "softening": "single_spline", "mem": "host pinned" if args.pinned else "host pageable",
+                   "positions": "posstruct (AoS)" if args.aos else "PosXYZ", "fused_kick": bool(args.fuse_kick)},
         "end_to_end": {
             "interactions_per_s": di / wall,
             "median_step_ms": 1e3 * float(np.median(walls)),
@@ -458,7 +478,7 @@ def run_stream(args):
         "autotune": [{k: e[k] for k in ("occupancy_quarters", "size_class", "sinks_per_lane", "uses")}
                      for e in table],
     }
-    if not args.no_cpu:
+    if not args.no_cpu and not args.fuse_kick:
         results = {"batched": {"acc_sample": {x: acc[x][:2000] for x in (sinks[0], sinks[len(sinks) // 2])}}}
         line["cpu_baseline"] = cpu_baseline(box, R, soft_len, args.cpu_seconds, results, slabs=sinks)
     print(json.dumps(line), flush=True)
