@@ -117,6 +117,43 @@ inline void *box_after(void *sources4, int32_t nsrc_blocks) {
 int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
                      const KickArgs &kick, void *stream);
 int device_cu_count();
+
+// ---- Device-side SIC planning (gpu_plan.hip) -----------------------------
+// Input: the cellinfo slabs of SlabId-R..SlabId+R in device memory (cpd*cpd
+// cells each, cell (y, z) at y*cpd + z) and the SIC's sink rows.
+struct GPlanIn {
+    int32_t cpd, R, W, jw, j_low;
+    const nf_cellinfo *ci[NF_MAX_WIDTH];
+    uint64_t nslab[NF_MAX_WIDTH];  // particles per slab: every planned cell must lie inside its slab
+};
+// Totals of phase A (device; atomically accumulated, so zeroed first).
+struct GPlanTotals {
+    uint64_t sink_total, source_total, direct_total, padded_direct_total;
+    uint64_t sink_blocks, source_blocks, chunks;
+    uint64_t items[3];  // work items for S = 1, 2, 4
+    uint64_t occ[4];    // sinks in pencils of <64, 64-255, 256-1023, >=1024 sinks
+    uint64_t bad;       // planned cells with a negative count or outside their slab
+};
+// Where phase B writes (the plan buffer at the SIC's PlanLayout offsets),
+// and its scratch of gpu_plan_scratch_ints() int32 (per-pencil counts and
+// their prefix sums, the work-list histogram).
+struct GPlanOut {
+    int32_t *sink_start, *sink_count, *parent, *src_start, *src_count, *src_parent, *ilist, *chunks;
+    uint32_t *idmax;
+    nf_sink_plan *sink_plan;
+    nf_source_plan *src_plan;
+    float *yoff;
+    int32_t sic;  // index of the SIC in its launch group (the work items' kind_sic)
+    int32_t *scratch;
+};
+constexpr int kPlanBuckets = 512;  // 1/8-octave cost buckets of a device-built work list
+size_t gpu_plan_scratch_ints(int nsink_sets, int nsrc_sets);
+// Phase A: the totals (sizes) of the SIC; the scratch must be the one phase
+// B gets.  Phase B: its arrays.  Items: the work list for sinks_per_lane S
+// (after phase B), longest first in 1/8-octave cost buckets.
+int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *scratch, void *stream);
+int launch_gpu_plan_b(const GPlanIn &in, const GPlanOut &out, void *stream);
+int launch_gpu_plan_items(const GPlanIn &in, const GPlanOut &out, int S, WorkItem *items, void *stream);
 // The occupancy target (waves per SIMD) the batched kernel is built with for
 // a pair form and tile; the third template argument of k_batched.
 int batched_kernel_waves(int softening, int sinks_per_lane);

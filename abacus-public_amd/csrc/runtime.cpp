@@ -84,6 +84,14 @@ using clk = std::chrono::steady_clock;
 
 struct Task {
     nf_sic sic;
+    // nf_submit_slab: the plan is computed on the device from the cellinfo
+    // slabs; sic holds the scalars (and, after phase A, the totals) only.
+    struct DevicePlanned {
+        bool on = false;
+        const nf_cellinfo *cellinfo[NF_MAX_WIDTH] = {};
+        uint64_t r0 = 0, r1 = 0;  // the SIC's sink rows in its slab
+        GPlanTotals tot{};        // phase A's sizes
+    } dp;
     std::atomic<int> state{0};  // 0 queued/running, 1 done, <0 failed
     std::string error;
     nf_task_stats stats{};
@@ -146,6 +154,10 @@ struct Worker {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     hipEvent_t plan_ready = nullptr;  // the plan image upload (copy stream) is done
+    hipStream_t pstream = nullptr;     // device planning (nf_submit_slab)
+    hipEvent_t gplan_ready = nullptr;  // the device-planned arrays and work lists are written
+    DevBuf gscratch;                   // the device planner's totals and scratch
+    PinBuf pin_tot;                    // phase A's totals, read back
     hipGraphExec_t gexec = nullptr;  // the last group's executable graph (use_graphs)
     uint64_t graphs_instantiated = 0, graphs_updated = 0;
     DevBuf plan, work, slabs;
@@ -183,8 +195,8 @@ struct Runtime {
     // Host-mode slabs resident in HBM.  A slot is reusable by address only
     // inside the stream session it was uploaded in (nf_stream_begin/end).
     struct Slot {
-        const float *host = nullptr;
-        uint64_t n = 0;
+        const void *host = nullptr;
+        uint64_t bytes = 0;
         uint64_t epoch = 0;
         bool valid = false;
         bool filling = false;  // an upload into it is being enqueued
@@ -229,7 +241,7 @@ int32_t count_coadd_chunks(const nf_sic &s) {
     return (int32_t)n;
 }
 
-PlanLayout plan_layout(const nf_sic &s, size_t base) {
+PlanLayout plan_layout(const nf_sic &s, size_t base, int32_t n_chunks) {
     PlanLayout L{};
     size_t o = base;
     auto take = [&](size_t bytes) {
@@ -248,10 +260,73 @@ PlanLayout plan_layout(const nf_sic &s, size_t base) {
     L.o_ilist = take(4 * (size_t)s.InteractionCount);
     L.o_yoff = take(4 * (size_t)s.InteractionCount);
     L.o_src_parent = take(4 * (size_t)s.NSourceBlocks);
-    L.n_chunks = count_coadd_chunks(s);
+    L.n_chunks = n_chunks;
     L.o_chunks = take(8 * (size_t)L.n_chunks);
     L.total = o - base;
     return L;
+}
+
+// Where the device planner writes a SIC's arrays: the plan buffer `base` at
+// the offsets of L.
+GPlanOut plan_out(char *base, const PlanLayout &L, int sic, int32_t *scratch) {
+    GPlanOut o{};
+    o.sink_start = reinterpret_cast<int32_t *>(base + L.o_sink_start);
+    o.sink_count = reinterpret_cast<int32_t *>(base + L.o_sink_count);
+    o.idmax = reinterpret_cast<uint32_t *>(base + L.o_idmax);
+    o.sink_plan = reinterpret_cast<nf_sink_plan *>(base + L.o_sink_plan);
+    o.src_start = reinterpret_cast<int32_t *>(base + L.o_src_start);
+    o.src_count = reinterpret_cast<int32_t *>(base + L.o_src_count);
+    o.src_plan = reinterpret_cast<nf_source_plan *>(base + L.o_src_plan);
+    o.parent = reinterpret_cast<int32_t *>(base + L.o_parent);
+    o.ilist = reinterpret_cast<int32_t *>(base + L.o_ilist);
+    o.yoff = reinterpret_cast<float *>(base + L.o_yoff);
+    o.src_parent = reinterpret_cast<int32_t *>(base + L.o_src_parent);
+    o.chunks = reinterpret_cast<int32_t *>(base + L.o_chunks);
+    o.sic = sic;
+    o.scratch = scratch;
+    return o;
+}
+
+// The scalar members of a SIC of slab `slab`, rows [j_low, j_high), as the
+// SetInteractionCollection constructor sets them (PencilOnPencil.cc:180-215)
+// for the non-PARALLEL layout.
+void sic_header(nf_sic &s, int cpd, int R, int slab, int j_low, int j_high, float eps, float b2, int softening) {
+    std::memset(&s, 0, sizeof(s));
+    s.SlabId = slab;
+    s.j_low = j_low;
+    s.j_high = j_high;
+    s.k_low = 0;
+    s.k_high = cpd;
+    s.Nk = cpd;
+    s.j_width = j_high - j_low;
+    s.cpd = cpd;
+    s.nfradius = R;
+    s.nfwidth = 2 * R + 1;
+    s.eps = eps;
+    s.b2 = b2;
+    s.softening = softening;
+    s.NSinkSets = s.j_width * s.Nk;
+    s.NSourceSets = (s.j_width + s.nfwidth - 1) * s.Nk;
+    s.InteractionCount = s.nfwidth * s.NSinkSets;
+}
+
+// The device planner's totals into the SIC's members.
+int apply_totals(nf_sic &s, const GPlanTotals &t) {
+    if (t.bad)
+        return fail(NF_ERR_INVALID, std::to_string(t.bad) + " planned cells have a negative count or lie outside their slab");
+    if (t.sink_blocks * NF_BLOCK_SIZE >= (uint64_t)INT32_MAX)
+        return fail(NF_ERR_CAPACITY, "device plan: padded sink count overflows int32");
+    if (t.source_blocks * NF_BLOCK_SIZE >= (uint64_t)INT32_MAX)
+        return fail(NF_ERR_CAPACITY, "device plan: padded source count overflows int32");
+    s.NSinkBlocks = (int32_t)t.sink_blocks;
+    s.NSourceBlocks = (int32_t)t.source_blocks;
+    s.DirectTotal = t.direct_total;
+    s.PaddedDirectTotal = t.padded_direct_total;
+    s.SinkTotal = t.sink_total;
+    s.SourceTotal = t.source_total;
+    s.PaddedSinkTotal = t.sink_blocks * NF_BLOCK_SIZE;
+    s.PaddedSourceTotal = t.source_blocks * NF_BLOCK_SIZE;
+    return NF_OK;
 }
 
 // Device working set of one SIC: sink stream, source stream + boxes, partials.
@@ -282,10 +357,18 @@ void sink_rows(const nf_sic &s, uint64_t *r0, uint64_t *r1) {
     const nf_sink_plan &last = s.SinkPlan[(size_t)(s.j_width - 1) * s.Nk + (s.Nk - 1)];
     *r1 = (uint64_t)last.cell[0].start + (uint64_t)last.cell[0].N;
 }
+void task_rows(const Task &t, uint64_t *r0, uint64_t *r1) {
+    if (t.dp.on) {
+        *r0 = t.dp.r0;
+        *r1 = t.dp.r1;
+    } else {
+        sink_rows(t.sic, r0, r1);
+    }
+}
 
 // Device bytes a SIC adds to a group (plan + work + host-mode slabs).
 size_t task_bytes(const nf_sic &s, int path) {
-    size_t b = plan_layout(s, 0).total + work_layout(s, path, 0).total + sizeof(SicView) + 256;
+    size_t b = plan_layout(s, 0, count_coadd_chunks(s)).total + work_layout(s, path, 0).total + sizeof(SicView) + 256;
     b += ((size_t)s.NSinkBlocks + 2 * (size_t)s.NSinkSets) * sizeof(WorkItem);  // at most one item per block + tails
     if (s.mem == NF_MEM_HOST) {
         for (int c = 0; c < s.nfwidth; c++) b += align_up(12 * (size_t)s.Nslab[c], 256);
@@ -433,6 +516,13 @@ std::string occupancy_key(const std::vector<Task *> &g) {
     uint64_t di = 0;
     for (Task *t : g) {
         di += t->sic.DirectTotal;
+        if (t->dp.on) {
+            for (int b = 0; b < 4; b++) {
+                h[b] += (double)t->dp.tot.occ[b];
+                tot += (double)t->dp.tot.occ[b];
+            }
+            continue;
+        }
         for (int k = 0; k < t->sic.NSinkSets; k++) {
             const int n = t->sic.SinkSetCount[k];
             h[n < 64 ? 0 : n < 256 ? 1 : n < 1024 ? 2 : 3] += n;
@@ -484,14 +574,14 @@ void par_memcpy(void *dst, const void *src, size_t bytes) {
 // directly.  The copy into staging runs outside the cache lock; a second user
 // of a slab being filled waits until its ready event is recorded, then waits
 // on that event on the GPU.
-int acquire_slab(Runtime &rt, const float *host, uint64_t n, Runtime::Slot **out, bool *uploaded) {
+int acquire_slab(Runtime &rt, const void *host, uint64_t bytes, Runtime::Slot **out, bool *uploaded) {
     auto &C = rt.cache;
     std::unique_lock<std::mutex> lk(C.m);
     *uploaded = false;
     if (C.active)
         for (auto &sp : C.slots) {
             Runtime::Slot &sl = *sp;
-            if ((sl.valid || sl.filling) && sl.host == host && sl.n == n && sl.epoch == C.epoch) {
+            if ((sl.valid || sl.filling) && sl.host == host && sl.bytes == bytes && sl.epoch == C.epoch) {
                 sl.refs++;
                 sl.last = ++C.clock;
                 C.cv.wait(lk, [&] { return !sl.filling; });
@@ -518,9 +608,8 @@ int acquire_slab(Runtime &rt, const float *host, uint64_t n, Runtime::Slot **out
     v->refs = 1;
     v->last = ++C.clock;
     v->host = host;
-    v->n = n;
+    v->bytes = bytes;
     v->epoch = C.epoch;
-    const size_t bytes = 12 * (size_t)n;
     auto finish_fill = [&](int rc, int k) {  // under lk
         v->filling = false;
         v->valid = rc == NF_OK;
@@ -575,6 +664,53 @@ void release_slab(Runtime &rt, Runtime::Slot *sl) {
     if (--sl->refs == 0 && !(C.active && sl->epoch == C.epoch)) sl->valid = false;
 }
 
+// Drops the slab-cache references a group holds when it is done.
+struct SlotRelease {
+    Runtime &rt;
+    std::vector<Runtime::Slot *> &v;
+    ~SlotRelease() {
+        for (auto *x : v)
+            if (x) release_slab(rt, x);
+    }
+};
+
+// Device planning, phase A, of a task from nf_submit_slab: its cellinfo
+// slabs go up through the slab cache (held in `cslot` until the group is
+// done), the planner's first kernels run on the worker's planning stream,
+// and the plan's sizes come back into t.sic.  Fills *in for phase B.
+int plan_phase_a(Runtime &rt, Worker &w, Task &t, std::vector<Runtime::Slot *> &cslot, GPlanIn *in,
+                 uint64_t *to_dev) {
+    nf_sic &s = t.sic;
+    std::memset(in, 0, sizeof(*in));
+    in->cpd = s.cpd;
+    in->R = s.nfradius;
+    in->W = s.nfwidth;
+    in->jw = s.j_width;
+    in->j_low = s.j_low;
+    const uint64_t cbytes = sizeof(nf_cellinfo) * (uint64_t)s.cpd * (uint64_t)s.cpd;
+    for (int c = 0; c < s.nfwidth; c++) {
+        Runtime::Slot *sl = nullptr;
+        bool up = false;
+        int rc = acquire_slab(rt, t.dp.cellinfo[c], cbytes, &sl, &up);
+        if (rc) return rc;
+        cslot.push_back(sl);
+        if (up) *to_dev += cbytes;
+        in->ci[c] = static_cast<const nf_cellinfo *>(sl->buf.p);
+        in->nslab[c] = s.Nslab[c];
+        NF_HIP(hipStreamWaitEvent(w.pstream, sl->ready, 0));
+    }
+    int rc = w.gscratch.ensure(256 + sizeof(int32_t) * gpu_plan_scratch_ints(s.NSinkSets, s.NSourceSets));
+    if (rc || (rc = w.pin_tot.ensure(sizeof(GPlanTotals)))) return rc;
+    GPlanTotals *tot = static_cast<GPlanTotals *>(w.gscratch.p);
+    int32_t *scratch = reinterpret_cast<int32_t *>(static_cast<char *>(w.gscratch.p) + 256);
+    const int e = launch_gpu_plan_a(*in, tot, scratch, w.pstream);
+    if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
+    NF_HIP(hipMemcpyAsync(w.pin_tot.p, tot, sizeof(GPlanTotals), hipMemcpyDeviceToHost, w.pstream));
+    NF_HIP(hipStreamSynchronize(w.pstream));
+    std::memcpy(&t.dp.tot, w.pin_tot.p, sizeof(GPlanTotals));
+    return apply_totals(s, t.dp.tot);
+}
+
 // Execute a group of compatible SICs on worker w: one plan upload, a gather
 // per SIC, one persistent direct-kernel launch over all their work items
 // (batched path; the plain path launches per SIC), a co-add per SIC.
@@ -584,6 +720,19 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     const int path = rt.params.path;
     TileShape ts = rt.shape;
     const int n = (int)g.size();
+    // A task from nf_submit_slab (always alone in its group) is planned on
+    // the device: phase A first, since everything below needs its sizes.
+    const bool dplan = g[0]->dp.on;
+    GPlanIn gin{};
+    std::vector<Runtime::Slot *> cslot;
+    SlotRelease cell_release{rt, cslot};
+    uint64_t cell_bytes = 0;
+    if (dplan) {
+        if (n != 1) return fail(NF_ERR_STATE, "a device-planned SIC must run alone");
+        const int rc = plan_phase_a(rt, w, *g[0], cslot, &gin, &cell_bytes);
+        if (rc) return rc;
+    }
+    const auto t_plan = clk::now();
     // Autotuning: a known occupancy histogram gets its tuned shape; a new one
     // is tuned on this group's launch (every candidate computes the same
     // partials; the last launch's are co-added).
@@ -636,15 +785,21 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
                 if (std::find(Svals.begin(), Svals.end(), c.sinks_per_lane) == Svals.end()) Svals.push_back(c.sinks_per_lane);
         if (std::find(Svals.begin(), Svals.end(), ts.sinks_per_lane) == Svals.end()) Svals.push_back(ts.sinks_per_lane);
     }
+    // (A device-planned SIC's lists are written on the device; phase A
+    // counted their items.)
     std::vector<std::vector<WorkItem>> itemsets;
-    for (int S : Svals) itemsets.push_back(make_items(S));
+    std::vector<size_t> nitems;
+    for (int S : Svals) {
+        itemsets.push_back(dplan ? std::vector<WorkItem>() : make_items(S));
+        nitems.push_back(dplan ? (size_t)g[0]->dp.tot.items[S == 1 ? 0 : S == 2 ? 1 : 2] : itemsets.back().size());
+    }
     auto items_of = [&](int S) -> int {
         for (size_t k = 0; k < Svals.size(); k++)
             if (Svals[k] == S) return (int)k;
         return 0;
     };
     size_t items_total = 0;
-    for (auto &v : itemsets) items_total += align_up(sizeof(WorkItem) * std::max<size_t>(v.size(), 1), 256);
+    for (size_t v : nitems) items_total += align_up(sizeof(WorkItem) * std::max<size_t>(v, 1), 256);
 
     // Plan upload: [views][items][counter][plan arrays of each SIC].
     const size_t o_views = 0;
@@ -654,14 +809,15 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         size_t o = o_items;
         for (size_t k = 0; k < itemsets.size(); k++) {
             o_itemset[k] = o;
-            o += align_up(sizeof(WorkItem) * std::max<size_t>(itemsets[k].size(), 1), 256);
+            o += align_up(sizeof(WorkItem) * std::max<size_t>(nitems[k], 1), 256);
         }
     }
     const size_t o_counter = align_up(o_items + std::max<size_t>(items_total, 256), 256);
     size_t plan_total = align_up(o_counter + 256, 256);
     std::vector<PlanLayout> PL(n);
     for (int i = 0; i < n; i++) {
-        PL[i] = plan_layout(g[i]->sic, plan_total);
+        PL[i] = plan_layout(g[i]->sic, plan_total,
+                            g[i]->dp.on ? (int32_t)g[i]->dp.tot.chunks : count_coadd_chunks(g[i]->sic));
         plan_total += PL[i].total;
     }
     // Device working set of each SIC.
@@ -692,25 +848,19 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     if (plan_total + work_total + slab_total > rt.max_buffer_bytes)
         return fail(NF_ERR_CAPACITY, "SIC group needs " + std::to_string(plan_total + work_total + slab_total) +
                                          " device bytes, buffer holds " + std::to_string(rt.max_buffer_bytes));
-    if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) || (rc = w.pin_plan.ensure(plan_total)))
+    if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) ||
+        (rc = w.pin_plan.ensure(dplan ? sizeof(SicView) : plan_total)))
         return rc;
     if (s0.mem == NF_MEM_HOST && (rc = w.slabs.ensure(slab_total))) return rc;
 
     // Slabs: HBM copies through the cache (uploads overlap other groups' kernels).
     std::vector<Runtime::Slot *> hslot(hslab.size(), nullptr);
-    struct SlotRelease {  // references are dropped when the group is done
-        Runtime &rt;
-        std::vector<Runtime::Slot *> &v;
-        ~SlotRelease() {
-            for (auto *x : v)
-                if (x) release_slab(rt, x);
-        }
-    } slot_release{rt, hslot};
+    SlotRelease slot_release{rt, hslot};
     int n_up = 0, n_reuse = 0;
     std::vector<char> fresh(hslab.size(), 0);
     for (size_t k = 0; k < hslab.size(); k++) {
         bool up = false;
-        if ((rc = acquire_slab(rt, hslab[k], hslab_n[k], &hslot[k], &up))) return rc;
+        if ((rc = acquire_slab(rt, hslab[k], 12 * hslab_n[k], &hslot[k], &up))) return rc;
         fresh[k] = up;
         if (up) n_up++;
         else n_reuse++;
@@ -729,32 +879,34 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     for (int i = 0; i < n; i++) {
         const nf_sic &s = g[i]->sic;
         const PlanLayout &L = PL[i];
-        std::memcpy(hp + L.o_sink_start, s.SinkSetStart, 4 * (size_t)s.NSinkSets);
-        std::memcpy(hp + L.o_sink_count, s.SinkSetCount, 4 * (size_t)s.NSinkSets);
-        std::memcpy(hp + L.o_idmax, s.SinkSetIdMax, 4 * (size_t)s.NSinkSets);
-        std::memcpy(hp + L.o_sink_plan, s.SinkPlan, sizeof(nf_sink_plan) * (size_t)s.NSinkSets);
-        std::memcpy(hp + L.o_src_start, s.SourceSetStart, 4 * (size_t)s.NSourceSets);
-        std::memcpy(hp + L.o_src_count, s.SourceSetCount, 4 * (size_t)s.NSourceSets);
-        std::memcpy(hp + L.o_src_plan, s.SourcePlan, sizeof(nf_source_plan) * (size_t)s.NSourceSets);
-        if (s.NSinkBlocks) std::memcpy(hp + L.o_parent, s.SinkBlockParentPencil, 4 * (size_t)s.NSinkBlocks);
-        std::memcpy(hp + L.o_ilist, s.SinkSourceInteractionList, 4 * (size_t)s.InteractionCount);
-        std::memcpy(hp + L.o_yoff, s.SinkSourceYOffset, 4 * (size_t)s.InteractionCount);
-        {   // block -> source pencil, and the co-add chunks
-            int32_t *sp = reinterpret_cast<int32_t *>(hp + L.o_src_parent);
-            std::fill(sp, sp + s.NSourceBlocks, -1);  // blocks of no pencil are skipped
-            for (int k = 0; k < s.NSourceSets; k++) {
-                const int b0 = s.SourceSetStart[k] / NF_BLOCK_SIZE;
-                const int nb = (s.SourceSetCount[k] + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE;
-                for (int b = 0; b < nb; b++) sp[b0 + b] = k;
-            }
-            int32_t *ch = reinterpret_cast<int32_t *>(hp + L.o_chunks);
-            int64_t q = 0;
-            for (int k = 0; k < s.NSinkSets; k++)
-                for (int f = 0; f < s.SinkPlan[k].cell[0].N; f += NF_BLOCK_SIZE) {
-                    ch[2 * q] = k;
-                    ch[2 * q + 1] = f;
-                    q++;
+        if (!g[i]->dp.on) {  // host-planned: copy the arrays, derive the lists
+            std::memcpy(hp + L.o_sink_start, s.SinkSetStart, 4 * (size_t)s.NSinkSets);
+            std::memcpy(hp + L.o_sink_count, s.SinkSetCount, 4 * (size_t)s.NSinkSets);
+            std::memcpy(hp + L.o_idmax, s.SinkSetIdMax, 4 * (size_t)s.NSinkSets);
+            std::memcpy(hp + L.o_sink_plan, s.SinkPlan, sizeof(nf_sink_plan) * (size_t)s.NSinkSets);
+            std::memcpy(hp + L.o_src_start, s.SourceSetStart, 4 * (size_t)s.NSourceSets);
+            std::memcpy(hp + L.o_src_count, s.SourceSetCount, 4 * (size_t)s.NSourceSets);
+            std::memcpy(hp + L.o_src_plan, s.SourcePlan, sizeof(nf_source_plan) * (size_t)s.NSourceSets);
+            if (s.NSinkBlocks) std::memcpy(hp + L.o_parent, s.SinkBlockParentPencil, 4 * (size_t)s.NSinkBlocks);
+            std::memcpy(hp + L.o_ilist, s.SinkSourceInteractionList, 4 * (size_t)s.InteractionCount);
+            std::memcpy(hp + L.o_yoff, s.SinkSourceYOffset, 4 * (size_t)s.InteractionCount);
+            {   // block -> source pencil, and the co-add chunks
+                int32_t *sp = reinterpret_cast<int32_t *>(hp + L.o_src_parent);
+                std::fill(sp, sp + s.NSourceBlocks, -1);  // blocks of no pencil are skipped
+                for (int k = 0; k < s.NSourceSets; k++) {
+                    const int b0 = s.SourceSetStart[k] / NF_BLOCK_SIZE;
+                    const int nb = (s.SourceSetCount[k] + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE;
+                    for (int b = 0; b < nb; b++) sp[b0 + b] = k;
                 }
+                int32_t *ch = reinterpret_cast<int32_t *>(hp + L.o_chunks);
+                int64_t q = 0;
+                for (int k = 0; k < s.NSinkSets; k++)
+                    for (int f = 0; f < s.SinkPlan[k].cell[0].N; f += NF_BLOCK_SIZE) {
+                        ch[2 * q] = k;
+                        ch[2 * q + 1] = f;
+                        q++;
+                    }
+            }
         }
 
         DevPlan &pl = plans[i];
@@ -799,7 +951,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         std::memset(&ka, 0, sizeof(ka));
         if (s.kick) {
             uint64_t r0, r1;
-            sink_rows(s, &r0, &r1);
+            task_rows(*g[i], &r0, &r1);
             ka.on = 1;
             ka.rescale = (float)s.kick->rescale;
             ka.inv_eps3 = (float)s.kick->inv_eps3;
@@ -827,8 +979,20 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     const auto t_fill = clk::now();
     // The plan image goes up on the copy stream, overlapping the kernels of
     // the group before (this worker's previous group has completed, so its
-    // plan buffer is free).
-    NF_HIP(hipMemcpyAsync(w.plan.p, hp, plan_total, hipMemcpyHostToDevice, rt.cache.up));
+    // plan buffer is free).  A device-planned SIC's image is its view only;
+    // phase B and the work lists fill the rest on the planning stream.
+    if (dplan) {
+        NF_HIP(hipMemcpyAsync(w.plan.p, hp, sizeof(SicView), hipMemcpyHostToDevice, w.pstream));
+        const GPlanOut go = plan_out(pd, PL[0], 0,
+                                     reinterpret_cast<int32_t *>(static_cast<char *>(w.gscratch.p) + 256));
+        int e = launch_gpu_plan_b(gin, go, w.pstream);
+        for (size_t k = 0; k < Svals.size() && !e; k++)
+            e = launch_gpu_plan_items(gin, go, Svals[k], reinterpret_cast<WorkItem *>(pd + o_itemset[k]), w.pstream);
+        if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
+        NF_HIP(hipEventRecord(w.gplan_ready, w.pstream));
+    } else {
+        NF_HIP(hipMemcpyAsync(w.plan.p, hp, plan_total, hipMemcpyHostToDevice, rt.cache.up));
+    }
     // Host mode with a fused kick: the far-field accelerations of the sink
     // rows go up beside the plan (pinned ones directly, else through staging).
     size_t far_bytes = 0;
@@ -836,7 +1000,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         const nf_sic &s = g[i]->sic;
         if (!s.kick || s.mem != NF_MEM_HOST) continue;
         uint64_t r0, r1;
-        sink_rows(s, &r0, &r1);
+        task_rows(*g[i], &r0, &r1);
         if (r1 > r0) far_bytes += 12 * (size_t)(r1 - r0);
     }
     if (far_bytes) {
@@ -846,7 +1010,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             const nf_sic &s = g[i]->sic;
             if (!s.kick || s.mem != NF_MEM_HOST) continue;
             uint64_t r0, r1;
-            sink_rows(s, &r0, &r1);
+            task_rows(*g[i], &r0, &r1);
             if (r1 <= r0) continue;
             const size_t nb = 12 * (size_t)(r1 - r0);
             const float *src = s.kick->FarAccSlab + 3 * r0;
@@ -864,7 +1028,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     hipStream_t stream = rt.shared ? rt.shared : w.stream;
     std::unique_lock<std::mutex> enqueue_lock(rt.enqueue, std::defer_lock);
     if (rt.shared) enqueue_lock.lock();
-    uint64_t to_dev = plan_total + far_bytes, from_dev = 0;
+    uint64_t to_dev = (dplan ? sizeof(SicView) : plan_total) + far_bytes + cell_bytes, from_dev = 0;
     // With graphs, the sequence below is captured and launched as one graph.
     const bool use_graph = rt.params.use_graphs && !trial;
     struct CaptureGuard {  // ends an open capture on every early return
@@ -878,6 +1042,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         }
     } capture{stream};
     NF_HIP(hipStreamWaitEvent(stream, w.plan_ready, 0));
+    if (dplan) NF_HIP(hipStreamWaitEvent(stream, w.gplan_ready, 0));
     for (size_t k = 0; k < hslot.size(); k++) NF_HIP(hipStreamWaitEvent(stream, hslot[k]->ready, 0));
     for (size_t k = 0; k < hslab.size(); k++)
         if (fresh[k]) to_dev += 12 * hslab_n[k];
@@ -901,7 +1066,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             const int k = items_of(shape.sinks_per_lane);
             return launch_batched(reinterpret_cast<const SicView *>(pd + o_views), s0.cpd, s0.softening, pc, shape,
                                   reinterpret_cast<const WorkItem *>(pd + o_itemset[k]),
-                                  (int32_t)itemsets[k].size(), counter, stream);
+                                  (int32_t)nitems[k], counter, stream);
         };
         if (trial) {
             // Time each candidate; the chosen shape runs last so its partials
@@ -988,7 +1153,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         for (int i = 0; i < n; i++) {
             const nf_sic &s = g[i]->sic;
             uint64_t r0, r1;
-            sink_rows(s, &r0, &r1);
+            task_rows(*g[i], &r0, &r1);
             if (r1 <= r0) continue;
             // The copy-back runs on the download stream, after the group's
             // kernels, while the next group's kernels use the compute stream:
@@ -1037,6 +1202,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         st.sinks_per_lane = path == NF_PATH_BATCHED ? ts.sinks_per_lane : 0;
         st.autotune_trial = trial ? 1 : 0;
         st.waves_per_eu = path == NF_PATH_BATCHED ? batched_kernel_waves(s0.softening, ts.sinks_per_lane) : 0;
+        st.device_planned = g[i]->dp.on ? 1 : 0;
+        st.plan_s = g[i]->dp.on ? std::chrono::duration<double>(t_plan - t0).count() : 0.0;
     }
     return NF_OK;
 }
@@ -1065,15 +1232,18 @@ void worker_main(Runtime *rt, Worker *w) {
             if (rt->queue.empty()) return;  // stopping and drained
             g.push_back(rt->queue.front());
             rt->queue.pop_front();
-            size_t bytes = task_bytes(g[0]->sic, rt->params.path);
+            // A device-planned SIC runs alone (its sizes are not known yet).
+            const bool solo = g[0]->dp.on;
+            size_t bytes = solo ? 0 : task_bytes(g[0]->sic, rt->params.path);
             // After the queue ran dry the GPU is idle: start it on a single SIC
             // instead of waiting for a whole group to be prepared.
-            const int limit = !rt->warm && rt->inflight == 0 ? 1 : rt->max_group;
+            const int limit = solo || (!rt->warm && rt->inflight == 0) ? 1 : rt->max_group;
             rt->warm = true;
             rt->inflight++;
             uint64_t work = g[0]->sic.DirectTotal;
             while (rt->params.path == NF_PATH_BATCHED && (int)g.size() < limit && work < rt->group_interactions &&
-                   !rt->queue.empty() && compatible(g[0]->sic, rt->queue.front()->sic)) {
+                   !rt->queue.empty() && !rt->queue.front()->dp.on &&
+                   compatible(g[0]->sic, rt->queue.front()->sic)) {
                 const size_t b = task_bytes(rt->queue.front()->sic, rt->params.path);
                 if (bytes + b > rt->max_buffer_bytes) break;
                 bytes += b;
@@ -1085,7 +1255,7 @@ void worker_main(Runtime *rt, Worker *w) {
         // Invalid descriptors fail alone; the rest of the group runs.
         std::vector<Task *> ok;
         for (Task *t : g) {
-            const int rc = validate(t->sic);
+            const int rc = t->dp.on ? NF_OK : validate(t->sic);  // nf_submit_slab checked its own
             if (rc) finish(*t, rc);
             else ok.push_back(t);
         }
@@ -1103,6 +1273,8 @@ int init_worker(Worker &w) {
     NF_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
     for (auto &e : w.ev) NF_HIP(hipEventCreate(&e));
     NF_HIP(hipEventCreateWithFlags(&w.plan_ready, hipEventDisableTiming));
+    NF_HIP(hipStreamCreateWithFlags(&w.pstream, hipStreamNonBlocking));
+    NF_HIP(hipEventCreateWithFlags(&w.gplan_ready, hipEventDisableTiming));
     w.ready = true;
     return NF_OK;
 }
@@ -1111,7 +1283,10 @@ void destroy_worker(Worker &w) {
     if (!w.ready) return;
     (void)hipSetDevice(w.device);
     if (w.stream) (void)hipStreamSynchronize(w.stream);
+    if (w.pstream) (void)hipStreamSynchronize(w.pstream);
     w.plan.release();
+    w.gscratch.release();
+    w.pin_tot.release();
     w.work.release();
     w.slabs.release();
     w.pin_plan.release();
@@ -1122,6 +1297,8 @@ void destroy_worker(Worker &w) {
     for (auto &e : w.ev)
         if (e) (void)hipEventDestroy(e);
     if (w.plan_ready) (void)hipEventDestroy(w.plan_ready);
+    if (w.gplan_ready) (void)hipEventDestroy(w.gplan_ready);
+    if (w.pstream) (void)hipStreamDestroy(w.pstream);
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.ready = false;
 }
@@ -1289,6 +1466,172 @@ extern "C" int nf_submit(const nf_sic *sic, int32_t blocking, nf_task **task) {
     }
     rt->qcv.notify_one();
     if (blocking) return nf_wait(*task);
+    return NF_OK;
+}
+
+namespace {
+
+// The checks nf_plan_sic makes of its arguments.
+int check_plan_args(int32_t cpd, int32_t R, int32_t j_low, int32_t j_high, const nf_cellinfo *const *cellinfo) {
+    if (!cellinfo) return fail(NF_ERR_INVALID, "null cellinfo");
+    if (cpd <= 0 || (cpd % 2) == 0) return fail(NF_ERR_INVALID, "cpd must be odd and positive");
+    if (R < 0 || R > NF_MAX_RADIUS || 2 * R + 1 > cpd) return fail(NF_ERR_INVALID, "nfradius out of range");
+    if (j_low < 0 || j_high > cpd || j_low >= j_high) return fail(NF_ERR_INVALID, "bad sink row range");
+    if ((int64_t)(j_high - j_low + 2 * R) * cpd * (2 * R + 1) >= INT32_MAX)
+        return fail(NF_ERR_CAPACITY, "too many pencils for int32 indices");
+    for (int c = 0; c < 2 * R + 1; c++)
+        if (!cellinfo[c]) return fail(NF_ERR_INVALID, "missing cellinfo slab");
+    return NF_OK;
+}
+
+}  // namespace
+
+extern "C" int nf_submit_slab(const nf_slab_task *p, int32_t blocking, nf_task **task) {
+    if (!p || !task) return fail(NF_ERR_INVALID, "null argument");
+    Runtime *rt;
+    {
+        std::lock_guard<std::mutex> lk(g_rt_mutex);
+        rt = g_rt;
+    }
+    if (!rt) return fail(NF_ERR_STATE, "nf_submit_slab before nf_setup");
+    int rc = check_plan_args(p->cpd, p->nfradius, p->j_low, p->j_high, p->cellinfo);
+    if (rc) return rc;
+    if (p->cpd != rt->params.cpd) return fail(NF_ERR_INVALID, "cpd differs from nf_setup's");
+    const int R = p->nfradius, W = 2 * R + 1;
+    if (p->mem != NF_MEM_HOST && p->mem != NF_MEM_DEVICE) return fail(NF_ERR_INVALID, "bad mem kind");
+    if (p->pos_layout != NF_POS_XYZ && p->pos_layout != NF_POS_AOS) return fail(NF_ERR_INVALID, "bad pos_layout");
+    if (!p->SinkAccSlab) return fail(NF_ERR_INVALID, "missing acceleration slab");
+    for (int c = 0; c < W; c++)
+        if (!p->PosSlab[c]) return fail(NF_ERR_INVALID, "missing position slab");
+    if (p->kick && (!p->kick->FarAccSlab || !std::isfinite(p->kick->rescale) || !std::isfinite(p->kick->inv_eps3) ||
+                    !std::isfinite(p->kick->density_rad2)))
+        return fail(NF_ERR_INVALID, "kick fusion needs a far-field slab and finite constants");
+    PairConst pc;
+    if ((rc = derive_pair_const(p->softening, p->eps, p->b2, &pc))) return rc;
+    Task *t = new Task();
+    nf_sic &s = t->sic;
+    sic_header(s, p->cpd, R, p->SlabId, p->j_low, p->j_high, p->eps, p->b2, p->softening);
+    s.mem = p->mem;
+    s.pos_layout = p->pos_layout;
+    s.kick = p->kick;
+    s.SinkPosSlab = p->PosSlab[R];
+    for (int c = 0; c < W; c++) {
+        s.SourcePosSlab[c] = p->PosSlab[c];
+        s.Nslab[c] = p->Nslab[c];
+    }
+    s.SinkAccSlab = p->SinkAccSlab;
+    t->dp.on = true;
+    for (int c = 0; c < W; c++) t->dp.cellinfo[c] = p->cellinfo[c];
+    // The sink rows run from cell (j_low, 0) to the end of cell (j_high-1,
+    // cpd-1); phase A checks both cells against the slab.
+    const nf_cellinfo &first = p->cellinfo[R][(size_t)p->j_low * p->cpd];
+    const nf_cellinfo &last = p->cellinfo[R][(size_t)(p->j_high - 1) * p->cpd + (p->cpd - 1)];
+    t->dp.r0 = (uint32_t)first.startindex;
+    t->dp.r1 = (uint64_t)(uint32_t)last.startindex + (uint64_t)std::max(last.count, 0);
+    *task = reinterpret_cast<nf_task *>(t);
+    {
+        std::lock_guard<std::mutex> q(rt->qm);
+        rt->queue.push_back(t);
+    }
+    rt->qcv.notify_one();
+    if (blocking) return nf_wait(*task);
+    return NF_OK;
+}
+
+extern "C" int nf_plan_sic_device(int32_t cpd, int32_t R, int32_t slab, int32_t j_low, int32_t j_high,
+                                  const nf_cellinfo *const *cellinfo, const float *const *pos_slab,
+                                  const uint64_t *nslab, float eps, float b2, int32_t softening, void *arena_ptr,
+                                  uint64_t arena_bytes, nf_sic *out) {
+    if (!out || !arena_ptr || !nslab) return fail(NF_ERR_INVALID, "nf_plan_sic_device: null argument");
+    int rc = check_plan_args(cpd, R, j_low, j_high, cellinfo);
+    if (rc) return rc;
+    const int W = 2 * R + 1;
+    nf_sic s;
+    sic_header(s, cpd, R, slab, j_low, j_high, eps, b2, softening);
+    s.mem = NF_MEM_HOST;
+    if (pos_slab) {
+        s.SinkPosSlab = pos_slab[R];
+        for (int c = 0; c < W; c++) s.SourcePosSlab[c] = pos_slab[c];
+    }
+    for (int c = 0; c < W; c++) s.Nslab[c] = nslab[c];
+
+    // Device buffers: the cellinfo slabs, the totals and scratch, the arrays.
+    struct Bufs {
+        void *cells = nullptr, *scr = nullptr, *plan = nullptr;
+        hipStream_t st = nullptr;
+        ~Bufs() {
+            if (st) (void)hipStreamSynchronize(st);
+            if (cells) (void)hipFree(cells);
+            if (scr) (void)hipFree(scr);
+            if (plan) (void)hipFree(plan);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } b;
+    const size_t cbytes = sizeof(nf_cellinfo) * (size_t)cpd * (size_t)cpd;
+    NF_HIP(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
+    NF_HIP(hipMalloc(&b.cells, cbytes * W));
+    NF_HIP(hipMalloc(&b.scr, 256 + sizeof(int32_t) * gpu_plan_scratch_ints(s.NSinkSets, s.NSourceSets)));
+    GPlanIn in{};
+    in.cpd = cpd;
+    in.R = R;
+    in.W = W;
+    in.jw = s.j_width;
+    in.j_low = j_low;
+    for (int c = 0; c < W; c++) {
+        in.ci[c] = reinterpret_cast<const nf_cellinfo *>(static_cast<char *>(b.cells) + cbytes * c);
+        in.nslab[c] = nslab[c];
+        NF_HIP(hipMemcpyAsync(const_cast<nf_cellinfo *>(in.ci[c]), cellinfo[c], cbytes, hipMemcpyHostToDevice, b.st));
+    }
+    GPlanTotals *tot_d = static_cast<GPlanTotals *>(b.scr);
+    int32_t *scratch = reinterpret_cast<int32_t *>(static_cast<char *>(b.scr) + 256);
+    int e = launch_gpu_plan_a(in, tot_d, scratch, b.st);
+    if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
+    GPlanTotals tot;
+    NF_HIP(hipMemcpyAsync(&tot, tot_d, sizeof(tot), hipMemcpyDeviceToHost, b.st));
+    NF_HIP(hipStreamSynchronize(b.st));
+    if ((rc = apply_totals(s, tot))) return rc;
+    const PlanLayout L = plan_layout(s, 0, (int32_t)tot.chunks);
+    NF_HIP(hipMalloc(&b.plan, L.total));
+    e = launch_gpu_plan_b(in, plan_out(static_cast<char *>(b.plan), L, 0, scratch), b.st);
+    if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
+
+    // The arrays, carved from the arena as nf_plan_sic carves them.
+    char *ap = static_cast<char *>(arena_ptr);
+    uint64_t left = arena_bytes;
+    auto take = [&](void **p, uint64_t bytes) {
+        const uint64_t mis = reinterpret_cast<uintptr_t>(ap) % 4096;
+        const uint64_t extra = mis ? 4096 - mis : 0;
+        if (left < bytes + extra) return false;
+        *p = ap + extra;
+        ap += extra + bytes;
+        left -= extra + bytes;
+        return true;
+    };
+    struct Part {
+        void **dst;
+        size_t bytes, off;
+    };
+    const size_t nss = (size_t)s.NSinkSets, nqs = (size_t)s.NSourceSets, ic = (size_t)s.InteractionCount;
+    const Part parts[] = {
+        {(void **)&s.SinkSetStart, 4 * nss, L.o_sink_start},
+        {(void **)&s.SinkSetCount, 4 * nss, L.o_sink_count},
+        {(void **)&s.SinkPlan, sizeof(nf_sink_plan) * nss, L.o_sink_plan},
+        {(void **)&s.SinkSetIdMax, 4 * nss, L.o_idmax},
+        {(void **)&s.SourceSetStart, 4 * nqs, L.o_src_start},
+        {(void **)&s.SourceSetCount, 4 * nqs, L.o_src_count},
+        {(void **)&s.SourcePlan, sizeof(nf_source_plan) * nqs, L.o_src_plan},
+        {(void **)&s.SinkBlockParentPencil, 4 * std::max<size_t>((size_t)s.NSinkBlocks, 1), L.o_parent},
+        {(void **)&s.SinkSourceInteractionList, 4 * ic, L.o_ilist},
+        {(void **)&s.SinkSourceYOffset, 4 * ic, L.o_yoff},
+    };
+    for (const Part &q : parts)
+        if (!take(q.dst, q.bytes)) return fail(NF_ERR_CAPACITY, "nf_plan_sic_device: arena too small");
+    for (const Part &q : parts) {
+        const size_t nb = q.dst == (void **)&s.SinkBlockParentPencil ? 4 * (size_t)s.NSinkBlocks : q.bytes;
+        if (nb) NF_HIP(hipMemcpyAsync(*q.dst, static_cast<char *>(b.plan) + q.off, nb, hipMemcpyDeviceToHost, b.st));
+    }
+    NF_HIP(hipStreamSynchronize(b.st));
+    *out = s;
     return NF_OK;
 }
 

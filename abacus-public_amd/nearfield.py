@@ -83,7 +83,17 @@ class nf_task_stats(C.Structure):
         ("group_direct_kernel_s", C.c_double), ("group_direct_total", C.c_uint64), ("graph", C.c_int32),
         ("upload_s", C.c_double), ("download_s", C.c_double), ("slabs_uploaded", C.c_int32),
         ("slabs_reused", C.c_int32), ("sinks_per_lane", C.c_int32), ("autotune_trial", C.c_int32),
-        ("waves_per_eu", C.c_int32),
+        ("waves_per_eu", C.c_int32), ("device_planned", C.c_int32), ("plan_s", C.c_double),
+    ]
+
+
+class nf_slab_task(C.Structure):
+    _fields_ = [
+        ("SlabId", C.c_int32), ("j_low", C.c_int32), ("j_high", C.c_int32), ("cpd", C.c_int32),
+        ("nfradius", C.c_int32), ("eps", C.c_float), ("b2", C.c_float), ("softening", C.c_int32),
+        ("mem", C.c_int32), ("pos_layout", C.c_int32), ("cellinfo", _P * NF_MAX_WIDTH),
+        ("PosSlab", _P * NF_MAX_WIDTH), ("Nslab", C.c_uint64 * NF_MAX_WIDTH), ("SinkAccSlab", _P),
+        ("kick", _P),
     ]
 
 
@@ -117,6 +127,8 @@ def load_library(path=LIB_PATH):
     lib.nf_plan_sic.argtypes = [I32, I32, I32, I32, I32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                 C.POINTER(U64), C.c_float, C.c_float, I32, C.c_void_p, U64,
                                 C.POINTER(nf_sic)]
+    lib.nf_plan_sic_device.argtypes = lib.nf_plan_sic.argtypes
+    lib.nf_submit_slab.argtypes = [C.POINTER(nf_slab_task), I32, C.POINTER(C.c_void_p)]
     lib.nf_device_alloc.argtypes = [U64, C.POINTER(C.c_void_p)]
     lib.nf_device_free.argtypes = [C.c_void_p]
     lib.nf_device_copy.argtypes = [C.c_void_p, C.c_void_p, U64, I32]
@@ -230,7 +242,10 @@ class SetInteractionCollection:
     """A planned SIC: the ``nf_sic`` descriptor and the arena behind it."""
 
     def __init__(self, box, slab, j_low, j_high, nfradius, eps, b2, softening,
-                 pos_ptrs=None, acc_ptr=None, mem=NF_MEM_HOST, pos_layout=NF_POS_XYZ, kick=None):
+                 pos_ptrs=None, acc_ptr=None, mem=NF_MEM_HOST, pos_layout=NF_POS_XYZ, kick=None,
+                 planner="host"):
+        """planner "host": nf_plan_sic; "device": nf_plan_sic_device (the
+        device planner nf_submit_slab uses, arrays copied back)."""
         cpd = box.cpd
         W = 2 * nfradius + 1
         slabs = [box.slab(slab - nfradius + c) for c in range(W)]
@@ -244,9 +259,10 @@ class SetInteractionCollection:
         nbytes = lib().nf_sic_arena_bytes(cpd, nfradius, j_low, j_high, maxn)
         self.arena = np.zeros(int(nbytes), np.uint8)
         self.desc = nf_sic()
-        check(lib().nf_plan_sic(cpd, nfradius, slab % cpd, j_low, j_high, ci, pos, nsl, float(eps),
-                                float(b2), SOFTENINGS[softening] if isinstance(softening, str) else softening,
-                                self.arena.ctypes.data, nbytes, C.byref(self.desc)), "nf_plan_sic")
+        fn = {"host": "nf_plan_sic", "device": "nf_plan_sic_device"}[planner]
+        check(getattr(lib(), fn)(cpd, nfradius, slab % cpd, j_low, j_high, ci, pos, nsl, float(eps),
+                                 float(b2), SOFTENINGS[softening] if isinstance(softening, str) else softening,
+                                 self.arena.ctypes.data, nbytes, C.byref(self.desc)), fn)
         self.desc.mem = mem
         if acc_ptr is not None:
             self.desc.SinkAccSlab = acc_ptr
@@ -365,15 +381,47 @@ class NearFieldDriver:
                 raise
         return t
 
-    def execute_slab(self, box, slab, acc, blocking=False, nsplit=1):
+    def submit_slab(self, box, slab, j_low=0, j_high=None, acc_ptr=None, pos_ptrs=None, mem=NF_MEM_HOST,
+                    pos_layout=NF_POS_XYZ, kick=None, blocking=False):
+        """nf_submit_slab: the SIC of ``slab``, rows [j_low, j_high), planned on
+        the device from the box's cellinfo slabs and executed."""
+        if j_high is None:
+            j_high = self.cpd
+        slabs = [box.slab(slab - self.R + c) for c in range(self.W)]
+        t = nf_slab_task(SlabId=slab % self.cpd, j_low=j_low, j_high=j_high, cpd=self.cpd, nfradius=self.R,
+                         eps=float(self.eps), b2=float(self.b2), softening=SOFTENINGS[self.softening] if isinstance(self.softening, str) else self.softening,
+                         mem=mem, pos_layout=pos_layout, SinkAccSlab=acc_ptr)
+        for c, sl in enumerate(slabs):
+            t.cellinfo[c] = sl.cellinfo.ctypes.data
+            t.PosSlab[c] = sl.pos.ctypes.data if pos_ptrs is None else pos_ptrs[c]
+            t.Nslab[c] = sl.n
+        if kick is not None:
+            t.kick = C.addressof(kick)
+        h = C.c_void_p()
+        check(lib().nf_submit_slab(C.byref(t), 0, C.byref(h)), "nf_submit_slab")
+        task = Task(h, (slabs, kick))  # the cellinfo, slabs and kick stay alive until completion
+        if blocking:
+            try:
+                task.wait()
+            except NFError:
+                task.release()
+                raise
+        return task
+
+    def execute_slab(self, box, slab, acc, blocking=False, nsplit=1, planner="host"):
         """ExecuteSlab for a host-resident box: plan the slab as ``nsplit`` SICs
         over y (directdriver.cpp:242-391) and submit them.  ``acc`` is the
-        float32 (n, 4) acceleration array of the slab, filled on completion."""
+        float32 (n, 4) acceleration array of the slab, filled on completion.
+        planner "device": each split is planned on the device (nf_submit_slab)."""
         cpd = self.cpd
         bounds = np.linspace(0, cpd, nsplit + 1).round().astype(int)
         tasks = []
         for jl, jh in zip(bounds[:-1], bounds[1:]):
             if jh <= jl:
+                continue
+            if planner == "device":
+                tasks.append(self.submit_slab(box, slab, int(jl), int(jh), acc_ptr=acc.ctypes.data,
+                                              blocking=blocking))
                 continue
             sic = self.plan(box, slab, int(jl), int(jh), acc_ptr=acc.ctypes.data)
             tasks.append(self.submit(sic, blocking))

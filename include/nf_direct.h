@@ -175,6 +175,9 @@ typedef struct nf_task_stats {
     int32_t sinks_per_lane;        /* batched path: register tile of its launch (0: plain) */
     int32_t autotune_trial;        /* 1: its launch group timed every candidate tile */
     int32_t waves_per_eu;          /* batched path: occupancy target its kernel was built with */
+    int32_t device_planned;        /* 1: submitted by nf_submit_slab (planned on the device) */
+    double plan_s;                 /* nf_submit_slab: worker wall time from the cellinfo
+                                      upload to the plan sizes being known on the host */
 } nf_task_stats;
 
 typedef struct nf_task nf_task;
@@ -226,6 +229,31 @@ int nf_reset(void);
  * queue (up to max_group) into one persistent launch.  The slabs and the plan
  * arrays must stay alive until the task completes. */
 int nf_submit(const nf_sic *sic, int32_t blocking, nf_task **task);
+
+/* ExecuteSlabGPU (directdriver.cpp:242-387) for one split of a slab, with the
+ * SetInteractionCollection constructor (PencilOnPencil.cc:180-418) run on the
+ * device instead of the host: the task plans the SIC of slab SlabId, sink
+ * rows [j_low, j_high), from the cellinfo slabs, then executes it as
+ * nf_submit does.  The plan arrays exist on the device only; the task's
+ * stats report the SIC's totals (direct_total ...).  The cellinfo slabs go up
+ * through the host-slab cache like position slabs (inside a stream session
+ * each is uploaded once).  A cell whose start + count falls outside its slab
+ * fails the task with NF_ERR_INVALID.  Such a task is launched on its own,
+ * never grouped with other SICs.  Every pointer must stay valid until the
+ * task completes. */
+typedef struct nf_slab_task {
+    int32_t SlabId, j_low, j_high, cpd, nfradius;
+    float eps, b2;                              /* as nf_sic */
+    int32_t softening;                          /* nf_softening */
+    int32_t mem;                                /* nf_mem of PosSlab, SinkAccSlab, kick->FarAccSlab */
+    int32_t pos_layout;                         /* nf_pos_layout of PosSlab */
+    const nf_cellinfo *cellinfo[NF_MAX_WIDTH];  /* host memory: slabs SlabId-R..SlabId+R, cpd*cpd cells, y-major */
+    const float *PosSlab[NF_MAX_WIDTH];         /* position slabs SlabId-R..SlabId+R */
+    uint64_t Nslab[NF_MAX_WIDTH];               /* particles per slab (with ghosts) */
+    float *SinkAccSlab;                         /* float4 per particle of slab SlabId */
+    const nf_kick *kick;                        /* optional, as nf_sic */
+} nf_slab_task;
+int nf_submit_slab(const nf_slab_task *t, int32_t blocking, nf_task **task);
 
 /* SetInteractionCollection::CheckCompletion (PencilOnPencil.cc:434-436):
  * 1 when done, 0 while running, < 0 if the task failed. */
@@ -301,6 +329,17 @@ int nf_plan_sic(int32_t cpd, int32_t nfradius, int32_t slab, int32_t j_low, int3
                 const nf_cellinfo *const *cellinfo, const float *const *pos_slab,
                 const uint64_t *nslab, float eps, float b2, int32_t softening, void *arena,
                 uint64_t arena_bytes, nf_sic *out);
+
+/* nf_plan_sic computed by the device planner nf_submit_slab uses (same
+ * arguments, same output; cellinfo in host memory; nslab must be given, as
+ * the device planner checks every cell against its slab).  Synchronous, on
+ * the current HIP device, with temporary device buffers.  No counterpart in
+ * the reference: it exposes the device planner for callers that keep their
+ * own plan arrays, and for checks against nf_plan_sic. */
+int nf_plan_sic_device(int32_t cpd, int32_t nfradius, int32_t slab, int32_t j_low, int32_t j_high,
+                       const nf_cellinfo *const *cellinfo, const float *const *pos_slab,
+                       const uint64_t *nslab, float eps, float b2, int32_t softening, void *arena,
+                       uint64_t arena_bytes, nf_sic *out);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,7 @@
 """Shared helpers for the near-field tests: workloads, error metrics."""
 import numpy as np
 
+import nearfield as nfm
 import pyoracle as po
 import synth
 
@@ -48,3 +49,59 @@ def not_soft_dominated(ref, soft):
 def oracle_slab(box, x, softening, eps, b2, with_soft=False):
     sl, ix, ce = po.slab_sinks(box, x)
     return po.direct_f64(box, R, softening, eps, b2, sl, ix, ce, with_soft=with_soft)
+
+
+def run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_XYZ, kick_args=None, planner="host",
+             with_stats=False):
+    """Whole-slab SICs for ``slabs`` in host or device mode; positions in the
+    given layout, an optional fused kick per slab (far-field slab, constants).
+    planner "host": nf_plan_sic + nf_submit; "device": nf_submit_slab.
+    Returns {slab: (n, 4) float32}; with_stats=True also the tasks' stats."""
+    cpd = box.cpd
+    keep, out, tasks, stats = [], {}, [], []
+    pos = {}
+    for x in range(cpd):
+        s = box.slab(x)
+        a = np.ascontiguousarray(s.xyz) if pos_layout == nfm.NF_POS_AOS else s.pos
+        if mem == nfm.NF_MEM_DEVICE:
+            d = nfm.DeviceArray(host=a)
+            keep.append(d)
+            pos[x] = d.ptr
+        else:
+            keep.append(a)
+            pos[x] = a.ctypes.data
+    for x in slabs:
+        n = box.slab(x).n
+        kick = None
+        if kick_args is not None:
+            far, rescale, inv_eps3, dens = kick_args[x]
+            if mem == nfm.NF_MEM_DEVICE:
+                fd = nfm.DeviceArray(host=far)
+                keep.append(fd)
+                fp = fd.ptr
+            else:
+                keep.append(far)
+                fp = far.ctypes.data
+            kick = nfm.nf_kick(FarAccSlab=fp, rescale=rescale, inv_eps3=inv_eps3, density_rad2=dens)
+        if mem == nfm.NF_MEM_DEVICE:
+            acc = nfm.DeviceArray(nbytes=16 * n)
+            keep.append(acc)
+            aptr = acc.ptr
+        else:
+            acc = np.full((n, 4), np.nan, np.float32)
+            aptr = acc.ctypes.data
+        ptrs = [pos[(x - R + c) % cpd] for c in range(2 * R + 1)]
+        if planner == "device":
+            tasks.append(drv.submit_slab(box, x, acc_ptr=aptr, pos_ptrs=ptrs, mem=mem, pos_layout=pos_layout,
+                                         kick=kick))
+        else:
+            sic = drv.plan(box, x, pos_ptrs=ptrs, acc_ptr=aptr, mem=mem, pos_layout=pos_layout, kick=kick)
+            tasks.append(drv.submit(sic))
+        out[x] = acc
+    for t in tasks:
+        t.wait()
+        stats.append(t.stats())
+        t.release()
+    res = {x: (a.to_host(np.float32, (box.slab(x).n, 4)) if mem == nfm.NF_MEM_DEVICE else a)
+           for x, a in out.items()}
+    return (res, stats) if with_stats else res

@@ -39,6 +39,10 @@ def test_struct_layout_matches_c(tmp_path):
            ' sizeof(nf_cellinfo), sizeof(nf_task_stats));']
     for f in fields:
         src.append(f'printf("%zu\\n", offsetof(nf_sic, {f}));')
+    slab_fields = [f for f, _ in nfm.nf_slab_task._fields_]
+    for f in slab_fields:
+        src.append(f'printf("%zu\\n", offsetof(nf_slab_task, {f}));')
+    src.append('printf("%zu\\n", sizeof(nf_slab_task));')
     src.append('printf("%zu %zu %zu\\n", offsetof(nf_kick, density_rad2), sizeof(nf_kick), sizeof(nf_setup_params));'
                ' return 0; }')
     c = tmp_path / "probe.c"
@@ -53,6 +57,9 @@ def test_struct_layout_matches_c(tmp_path):
     assert sizes[1] == 44 and sizes[2] == 40
     offs = list(map(int, out[5:5 + len(fields)]))
     assert offs == [getattr(nfm.nf_sic, f).offset for f in fields]
+    k = 5 + len(fields)
+    offs = list(map(int, out[k:k + len(slab_fields) + 1]))
+    assert offs == [getattr(nfm.nf_slab_task, f).offset for f in slab_fields] + [C.sizeof(nfm.nf_slab_task)]
     assert list(map(int, out[-3:])) == [nfm.nf_kick.density_rad2.offset, C.sizeof(nfm.nf_kick),
                                         C.sizeof(nfm.nf_setup_params)]
 
@@ -75,12 +82,22 @@ def test_plan_errors_are_codes():
     assert lib.nf_plan_sic(7, R, 0, 0, 7, ci, None, None, eps, b2, 1, arena.ctypes.data, arena.size,
                            C.byref(out)) == 0
     assert out.NSinkSets == 49 and out.InteractionCount == 5 * 49
+    # the device planner rejects the same arguments before touching a GPU
+    nsl = (C.c_uint64 * W)(*[box.slab(c).n for c in range(W)])
+    assert lib.nf_plan_sic_device(8, R, 0, 0, 8, ci, None, nsl, eps, b2, 1, arena.ctypes.data, arena.size,
+                                  C.byref(out)) == -1
+    assert lib.nf_plan_sic_device(7, R, 0, 5, 3, ci, None, nsl, eps, b2, 1, arena.ctypes.data, arena.size,
+                                  C.byref(out)) == -1
+    assert lib.nf_plan_sic_device(7, R, 0, 0, 7, ci, None, None, eps, b2, 1, arena.ctypes.data, arena.size,
+                                  C.byref(out)) == -1
 
 
 def test_runtime_errors_without_setup():
     lib = nfm.lib()
     h = C.c_void_p()
     assert lib.nf_submit(C.byref(nfm.nf_sic()), 0, C.byref(h)) == -4
+    assert lib.nf_submit_slab(C.byref(nfm.nf_slab_task()), 0, C.byref(h)) == -4
+    assert lib.nf_submit_slab(None, 0, C.byref(h)) == -1
     assert lib.nf_reset() == -4
     assert lib.nf_wait(None) == -1
     bad = nfm.nf_setup_params(cpd=7, max_buffer_bytes=1 << 30, n_gpu=2, buffers_per_device=1, path=1)

@@ -9,7 +9,8 @@ import pytest
 import nearfield as nfm
 import pyoracle as po
 import synth
-from nfhelpers import R, not_soft_dominated, oracle_slab, params, rel_err, small_uniform, summarize
+from nfhelpers import (R, not_soft_dominated, oracle_slab, params, rel_err, run_sics, small_uniform,
+                       summarize)
 
 pytestmark = pytest.mark.gpu
 
@@ -340,54 +341,6 @@ def test_pinned_host_slabs_match_pageable(box9):
         assert np.array_equal(out["pageable"][x], out["pinned"][x]), x
 
 
-def _run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_XYZ, kick_args=None):
-    """Whole-slab SICs for ``slabs`` in host or device mode; positions in the
-    given layout, an optional fused kick per slab (far-field slab, constants).
-    Returns {slab: (n, 4) float32}."""
-    cpd = box.cpd
-    keep, out, tasks = [], {}, []
-    pos = {}
-    for x in range(cpd):
-        s = box.slab(x)
-        a = np.ascontiguousarray(s.xyz) if pos_layout == nfm.NF_POS_AOS else s.pos
-        if mem == nfm.NF_MEM_DEVICE:
-            d = nfm.DeviceArray(host=a)
-            keep.append(d)
-            pos[x] = d.ptr
-        else:
-            keep.append(a)
-            pos[x] = a.ctypes.data
-    for x in slabs:
-        n = box.slab(x).n
-        kick = None
-        if kick_args is not None:
-            far, rescale, inv_eps3, dens = kick_args[x]
-            if mem == nfm.NF_MEM_DEVICE:
-                fd = nfm.DeviceArray(host=far)
-                keep.append(fd)
-                fp = fd.ptr
-            else:
-                keep.append(far)
-                fp = far.ctypes.data
-            kick = nfm.nf_kick(FarAccSlab=fp, rescale=rescale, inv_eps3=inv_eps3, density_rad2=dens)
-        if mem == nfm.NF_MEM_DEVICE:
-            acc = nfm.DeviceArray(nbytes=16 * n)
-            keep.append(acc)
-            aptr = acc.ptr
-        else:
-            acc = np.full((n, 4), np.nan, np.float32)
-            aptr = acc.ctypes.data
-        ptrs = [pos[(x - R + c) % cpd] for c in range(2 * R + 1)]
-        sic = drv.plan(box, x, pos_ptrs=ptrs, acc_ptr=aptr, mem=mem, pos_layout=pos_layout, kick=kick)
-        tasks.append(drv.submit(sic))
-        out[x] = acc
-    for t in tasks:
-        t.wait()
-        t.release()
-    return {x: (a.to_host(np.float32, (box.slab(x).n, 4)) if mem == nfm.NF_MEM_DEVICE else a)
-            for x, a in out.items()}
-
-
 @pytest.mark.parametrize("mem", [nfm.NF_MEM_HOST, nfm.NF_MEM_DEVICE])
 @pytest.mark.parametrize("path", ["plain", "batched"])
 def test_aos_positions_match_posxyz(box9, path, mem):
@@ -397,8 +350,8 @@ def test_aos_positions_match_posxyz(box9, path, mem):
     slabs = [0, 4, 8]
     with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path=path, buffers_per_device=2,
                              max_buffer_bytes=2 << 30, sinks_per_lane=2) as drv:
-        soa = _run_sics(drv, box9, slabs, mem)
-        aos = _run_sics(drv, box9, slabs, mem, pos_layout=nfm.NF_POS_AOS)
+        soa = run_sics(drv, box9, slabs, mem)
+        aos = run_sics(drv, box9, slabs, mem, pos_layout=nfm.NF_POS_AOS)
     for x in slabs:
         assert np.isfinite(soa[x]).all()
         assert np.array_equal(soa[x], aos[x]), x
@@ -420,8 +373,8 @@ def test_fused_kick_matches_consumer(box9, path, mem):
                  for x in slabs}
     with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path=path, buffers_per_device=2,
                              max_buffer_bytes=2 << 30, sinks_per_lane=2) as drv:
-        near = _run_sics(drv, box9, slabs, mem)
-        fused = _run_sics(drv, box9, slabs, mem, kick_args=kick_args)
+        near = run_sics(drv, box9, slabs, mem)
+        fused = run_sics(drv, box9, slabs, mem, kick_args=kick_args)
     for x in slabs:
         expect = po.rescale_coadd(near[x], kick_args[x][0], rescale, inv_eps3, dens)
         assert np.array_equal(fused[x], expect), x
