@@ -1,15 +1,26 @@
 // Device-side SIC planning: the SetInteractionCollection constructor
 // (PencilOnPencil.cc:180-418, non-PARALLEL layout) computed on the GPU from
-// the W slabs' cellinfo, plus the runtime's derived
-// lists (the source blocks' parent pencils, the co-add chunks, the batched
-// work list).  Every array has exactly the contents the host planner
-// (planner.cpp) and the runtime's host code give it, except the work list,
-// whose order within a cost bucket may differ (each sink belongs to exactly
-// one item, so the results do not depend on the order).
+// the W slabs' cellinfo, plus the runtime's derived lists (the source blocks'
+// parent pencils, the co-add chunks, the batched work list).  Every array has
+// exactly the contents the host planner (planner.cpp) and the runtime's host
+// code give it, except the work list, whose order within a cost bucket may
+// differ (each sink belongs to exactly one item, so the results do not depend
+// on the order).
 //
-// Phase A (launch_gpu_plan_a) accumulates the sizes the host needs to lay
-// the plan out; phase B (launch_gpu_plan_b) writes the arrays by prefix sums
-// over the pencils; launch_gpu_plan_items writes one work list.
+// Four launches per SIC, none with a per-thread loop longer than the pencil
+// width (items are counted in closed form, and a pencil's full tiles are
+// placed as one contiguous run):
+//   A0  zero the totals (and, for the first SIC of a group, the work-list
+//       histogram and cursors);
+//   A   per pencil: counts, blocks, chunks, items, pair totals, cell checks
+//       -> totals (atomics); the host reads them to lay the plan out;
+//   B1  sink pencils: SinkPlan, SinkSetCount, the work-list histogram;
+//       source pencils: SourcePlan, SourceSetCount; and per 256-pencil
+//       block the sums of padded blocks / chunks (for the prefix sums);
+//   B2  each workgroup adds up the block sums before it, scans its own
+//       pencils, and writes the starts, IdMax, block parents, interaction
+//       list, y offsets, co-add chunks and its pencils' work items (bucket
+//       offsets from an in-LDS scan of the histogram).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -20,21 +31,12 @@ namespace nf {
 namespace {
 
 constexpr int kB = NF_BLOCK_SIZE;
-constexpr int kScanThreads = 1024;
+constexpr int kT = 256;  // threads per workgroup of the planner kernels
+static_assert(kPlanBuckets == 2 * kT, "k_plan_b2 scans the histogram two buckets per thread");
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
 __device__ __forceinline__ int nblocks(int n) { return (n + kB - 1) / kB; }
-
-// Scratch regions (int32 offsets) for P sink pencils and Q source pencils.
-struct ScratchMap {
-    int sblk, sscan;  // padded blocks per sink pencil, and their exclusive scan
-    int qblk, qscan;  // the same per source pencil
-    int cblk, cscan;  // co-add chunks per sink pencil, and their scan
-    int hist;         // work-list bucket histogram, offsets, cursors
-};
-__host__ __device__ inline ScratchMap scratch_map(int P, int Q) {
-    return ScratchMap{0, P, 2 * P, 2 * P + Q, 2 * P + 2 * Q, 3 * P + 2 * Q, 4 * P + 2 * Q};
-}
+__host__ __device__ inline int ngroups(int n) { return (n + kT - 1) / kT; }
 
 // Sum over the wave, then one atomic per wave.
 __device__ __forceinline__ void wave_add(uint64_t *p, uint64_t v) {
@@ -50,49 +52,46 @@ __device__ __forceinline__ bool cell_ok(uint64_t start, int32_t n, uint64_t nsla
     return n >= 0 && (uint64_t)(uint32_t)start + (uint64_t)n <= nslab;
 }
 
-// Sinks of sink pencil (j, k): cells (j + j_low, k + c) of the central slab;
-// *bad counts its cells outside the slab.
-__device__ __forceinline__ int sink_pencil_count(const GPlanIn &in, int j, int k, uint64_t *bad) {
-    const nf_cellinfo *row = in.ci[in.R] + (size_t)(j + in.j_low) * in.cpd;
-    int n = 0;
-    for (int c = 0; c < in.W; c++) {
-        const nf_cellinfo &ci = row[wrapi(k + c, in.cpd)];
-        n += ci.count;
-        *bad += cell_ok(ci.startindex, ci.count, in.nslab[in.R]) ? 0 : 1;
-    }
-    return n;
+// Row of sink pencil (j, k) in the central slab: cells (j + j_low, k + c).
+__device__ __forceinline__ const nf_cellinfo *sink_row(const GPlanIn &in, int j) {
+    return in.ci[in.R] + (size_t)(j + in.j_low) * in.cpd;
 }
-// Row and column of source pencil (jj, k): y = jj + j_low - R (periodic),
-// z = zcen(k) = k + R (periodic), one cell in each of the W slabs.
+// Cell of source pencil (jj, k) in each slab: y = jj + j_low - R, z =
+// zcen(k) = k + R (both periodic).
 __device__ __forceinline__ int source_cell(const GPlanIn &in, int jj, int k) {
     return wrapi(jj + in.j_low - in.R, in.cpd) * in.cpd + wrapi(k + in.R, in.cpd);
 }
-__device__ __forceinline__ int source_pencil_count(const GPlanIn &in, int jj, int k, uint64_t *bad) {
+__device__ __forceinline__ int source_pencil_count(const GPlanIn &in, int jj, int k) {
     const int cell = source_cell(in, jj, k);
     int n = 0;
-    for (int c = 0; c < in.W; c++) {
-        const nf_cellinfo &ci = in.ci[c][cell];
-        n += ci.count;
-        *bad += cell_ok(ci.startindex_with_ghost, ci.count, in.nslab[c]) ? 0 : 1;
-    }
+    for (int c = 0; c < in.W; c++) n += in.ci[c][cell].count;
+    return n;
+}
+// Sources sink pencil (j, k) meets: its W source pencils (j + y, k).
+__device__ __forceinline__ uint64_t sink_pencil_sources(const GPlanIn &in, int j, int k) {
+    uint64_t n = 0;
+    for (int y = 0; y < in.W; y++) n += (uint64_t)source_pencil_count(in, j + y, k);
     return n;
 }
 
-// Work items of a pencil of n sinks for register tile S (the runtime's host
-// build_items): full tiles of 64 S, one-wave tiles while they keep at least
-// kTailLanes lanes busy, then one short tail.  f(first, nsink, kind).
-template <typename F>
-__device__ __forceinline__ void for_items(int n, int S, F f) {
-    const int tile = kB * S;
-    int first = 0;
-    for (; first + tile <= n; first += tile) f(first, tile, 0);
-    while (n - first >= kTailLanes) {
-        const int m = min(kB, n - first);
-        f(first, m, 2);
-        first += m;
+// The work items of a pencil of n sinks for register tile S, as the runtime's
+// host build_items makes them: `full` tiles of 64 S sinks (kind 0), then
+// one-wave tiles (kind 2) while at least kTailLanes sinks remain -- q of 64
+// sinks and, if the rest s is at least kTailLanes, one of s -- then one short
+// tail of s sinks (kind 1) if 0 < s < kTailLanes.
+struct ItemSplit {
+    int full, tile, n2, q, s, tail;
+    __device__ ItemSplit(int n, int S) {
+        tile = kB * S;
+        full = n / tile;
+        const int r = n - full * tile;
+        q = r / kB;
+        s = r - q * kB;
+        n2 = q + (s >= kTailLanes ? 1 : 0);
+        tail = (s > 0 && s < kTailLanes) ? 1 : 0;
     }
-    if (first < n) f(first, n - first, 1);
-}
+    __device__ int count() const { return full + n2 + tail; }
+};
 
 // 1/8-octave cost buckets, longest first (cost 0 last).
 __device__ __forceinline__ int cost_bucket(uint64_t cost) {
@@ -101,38 +100,83 @@ __device__ __forceinline__ int cost_bucket(uint64_t cost) {
     return kPlanBuckets - 1 - min(max(b, 0), kPlanBuckets - 1);
 }
 
-// ---- Phase A ---------------------------------------------------------------
+// ---- Workgroup scans (kT threads = 4 waves) --------------------------------
 
-__global__ void k_zero_totals(GPlanTotals *tot) {
-    uint64_t *p = reinterpret_cast<uint64_t *>(tot);
-    constexpr int n = sizeof(GPlanTotals) / sizeof(uint64_t);
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+// Exclusive prefix of v over the workgroup; *total gets the sum.  Every
+// thread must call it.
+__device__ __forceinline__ int block_excl_scan(int v, int *total) {
+    __shared__ int wsum[kT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(x, off);
+        if (lane >= off) x += u;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < kT / 64; w++) {
+        before += w < wave ? wsum[w] : 0;
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
 }
 
-// Threads [0, P): sink pencils; [P, P+Q): source pencils.  Counts go to the
-// sblk / qblk regions (phase B overwrites them), totals by atomics.
-__global__ __launch_bounds__(256) void k_plan_a_counts(GPlanIn in, GPlanTotals *tot, int32_t *scratch) {
+// Sum of part[0, n) over the workgroup (every thread gets it).
+__device__ __forceinline__ int block_sum_prefix(const int32_t *part, int n) {
+    int v = 0;
+    for (int i = threadIdx.x; i < n; i += kT) v += part[i];
+    int total;
+    (void)block_excl_scan(v, &total);
+    return total;
+}
+
+// ---- Phase A ---------------------------------------------------------------
+
+__global__ void k_plan_zero(GPlanTotals *tot, int32_t *shared, int nshared) {
+    uint64_t *p = reinterpret_cast<uint64_t *>(tot);
+    constexpr int n = sizeof(GPlanTotals) / sizeof(uint64_t);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+    for (int i = threadIdx.x; i < nshared; i += blockDim.x) shared[i] = 0;
+}
+
+// Threads [0, P): sink pencils; [P, P+Q): source pencils.
+__global__ __launch_bounds__(kT) void k_plan_a(GPlanIn in, GPlanTotals *tot) {
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const ScratchMap m = scratch_map(P, Q);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t sink = 0, sblk = 0, ch = 0, it[3] = {0, 0, 0}, occ[4] = {0, 0, 0, 0}, src = 0, qblk = 0, bad = 0;
+    uint64_t sink = 0, sblk = 0, ch = 0, it[3] = {0, 0, 0}, occ[4] = {0, 0, 0, 0}, src = 0, qblk = 0, bad = 0,
+             d = 0, pd = 0;
     if (t < P) {
         const int j = t / in.cpd, k = t % in.cpd;
-        const int n = sink_pencil_count(in, j, k, &bad);
-        scratch[m.sblk + t] = n;
+        const nf_cellinfo *row = sink_row(in, j);
+        int n = 0;
+        for (int c = 0; c < in.W; c++) {
+            const nf_cellinfo &ci = row[wrapi(k + c, in.cpd)];
+            n += ci.count;
+            bad += cell_ok(ci.startindex, ci.count, in.nslab[in.R]) ? 0 : 1;
+        }
         sink = (uint64_t)n;
         sblk = (uint64_t)nblocks(n);
-        // Pencil k starts with cell z = k: its chunks are that cell's.
-        ch = (uint64_t)nblocks(in.ci[in.R][(size_t)(j + in.j_low) * in.cpd + k].count);
-        for_items(n, 1, [&](int, int, int) { it[0]++; });
-        for_items(n, 2, [&](int, int, int) { it[1]++; });
-        for_items(n, 4, [&](int, int, int) { it[2]++; });
+        ch = (uint64_t)nblocks(row[k].count);  // pencil k starts with cell z = k: its chunks are that cell's
+        it[0] = (uint64_t)ItemSplit(n, 1).count();
+        it[1] = (uint64_t)ItemSplit(n, 2).count();
+        it[2] = (uint64_t)ItemSplit(n, 4).count();
         const int bin = n < 64 ? 0 : n < 256 ? 1 : n < 1024 ? 2 : 3;
         for (int i = 0; i < 4; i++) occ[i] = bin == i ? (uint64_t)n : 0;
+        const uint64_t nsrc = sink_pencil_sources(in, j, k);
+        d = (uint64_t)n * nsrc;
+        pd = sblk * kB * nsrc;
     } else if (t < P + Q) {
         const int u = t - P;
-        const int n = source_pencil_count(in, u / in.cpd, u % in.cpd, &bad);
-        scratch[m.qblk + u] = n;
+        const int cell = source_cell(in, u / in.cpd, u % in.cpd);
+        int n = 0;
+        for (int c = 0; c < in.W; c++) {
+            const nf_cellinfo &ci = in.ci[c][cell];
+            n += ci.count;
+            bad += cell_ok(ci.startindex_with_ghost, ci.count, in.nslab[c]) ? 0 : 1;
+        }
         src = (uint64_t)n;
         qblk = (uint64_t)nblocks(n);
     }
@@ -144,234 +188,226 @@ __global__ __launch_bounds__(256) void k_plan_a_counts(GPlanIn in, GPlanTotals *
     wave_add(&tot->source_total, src);
     wave_add(&tot->source_blocks, qblk);
     wave_add(&tot->bad, bad);
-}
-
-// DirectTotal and PaddedDirectTotal over each sink pencil's W source pencils
-// (j + y, k), y = 0..W-1.
-__global__ __launch_bounds__(256) void k_plan_a_direct(GPlanIn in, GPlanTotals *tot, const int32_t *scratch) {
-    const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const ScratchMap m = scratch_map(P, Q);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t d = 0, pd = 0;
-    if (t < P) {
-        const int j = t / in.cpd, k = t % in.cpd;
-        const int ns = scratch[m.sblk + t];
-        uint64_t nsrc = 0;
-        for (int y = 0; y < in.W; y++) nsrc += (uint64_t)scratch[m.qblk + (j + y) * in.cpd + k];
-        d = (uint64_t)ns * nsrc;
-        pd = (uint64_t)nblocks(ns) * kB * nsrc;
-    }
     wave_add(&tot->direct_total, d);
     wave_add(&tot->padded_direct_total, pd);
 }
 
 // ---- Phase B ---------------------------------------------------------------
 
-// Exclusive prefix sum of n int32 (n of a few 10^5 at most) by one workgroup:
-// each thread sums a contiguous run, a scan over the run sums, each thread
-// writes its run.
-__global__ __launch_bounds__(kScanThreads) void k_scan_excl(const int32_t *in, int32_t *out, int n) {
-    __shared__ int64_t part[kScanThreads];
-    const int t = threadIdx.x;
-    const int per = (n + kScanThreads - 1) / kScanThreads;
-    const int b = min(n, t * per), e = min(n, b + per);
-    int64_t s = 0;
-    for (int i = b; i < e; i++) s += in[i];
-    part[t] = s;
+// Workgroups [0, GP): sink pencils kT at a time; [GP, GP+GQ): source pencils.
+__global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) {
+    const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
+    const int GP = ngroups(P);
+    if ((int)blockIdx.x < GP) {
+        // The work-list histogram is built in LDS, then added to the group's
+        // with one atomic per workgroup and bucket: nearly every pencil of a
+        // uniform slab falls in the same bucket.
+        __shared__ int32_t lh[3 * kPlanBuckets];
+        for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT) lh[i] = 0;
+        __syncthreads();
+        const int s = blockIdx.x * kT + threadIdx.x;
+        int nb = 0, nc = 0;
+        if (s < P) {
+            const int j = s / in.cpd, k = s % in.cpd;
+            const nf_cellinfo *row = sink_row(in, j);
+            nf_sink_plan pl;
+            int n = 0;
+            for (int c = 0; c < NF_MAX_WIDTH; c++) {
+                if (c < in.W) {
+                    const nf_cellinfo &ci = row[wrapi(k + c, in.cpd)];
+                    pl.cell[c].start = (uint32_t)ci.startindex;
+                    pl.cell[c].N = ci.count;
+                    n += ci.count;
+                } else {
+                    pl.cell[c].start = 0;
+                    pl.cell[c].N = 0;
+                }
+            }
+            // The cell that sits at z = 0 after the periodic wrap
+            // (PencilPlan.cc:123-128): k + c = cpd, if within the pencil.
+            const int wc = (in.cpd - k) % in.cpd;
+            pl.wrapcell = wc < in.W ? wc : 0;
+            out.sink_plan[s] = pl;
+            out.sink_count[s] = n;
+            nb = nblocks(n);
+            nc = nblocks(pl.cell[0].N);
+            if (n > 0) {  // the work-list histogram, per wanted tile
+                const uint64_t nsrc = sink_pencil_sources(in, j, k);
+                for (int ti = 0; ti < 3; ti++) {
+                    if (!out.items[ti]) continue;
+                    const ItemSplit sp(n, 1 << ti);
+                    int32_t *h = lh + ti * kPlanBuckets;
+                    if (sp.full) atomicAdd(h + cost_bucket((uint64_t)sp.tile * nsrc), sp.full);
+                    if (sp.n2) atomicAdd(h + cost_bucket((uint64_t)kB * nsrc), sp.n2);
+                    if (sp.tail) atomicAdd(h + cost_bucket((uint64_t)sp.s * nsrc), 1);
+                }
+            }
+        }
+        int tb, tc;
+        (void)block_excl_scan(nb, &tb);  // (its barriers also order the LDS histogram)
+        (void)block_excl_scan(nc, &tc);
+        if (threadIdx.x == 0) {
+            out.part[blockIdx.x] = tb;
+            out.part[GP + blockIdx.x] = tc;
+        }
+        for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT)
+            if (lh[i]) atomicAdd(out.hist + i, lh[i]);
+    } else {
+        const int g = blockIdx.x - GP;
+        const int u = g * kT + threadIdx.x;
+        int nb = 0;
+        if (u < Q) {
+            const int cell = source_cell(in, u / in.cpd, u % in.cpd);
+            nf_source_plan pl;
+            int n = 0;
+            for (int c = 0; c < NF_MAX_WIDTH; c++) {
+                if (c < in.W) {
+                    const nf_cellinfo &ci = in.ci[c][cell];
+                    pl.cell[c].start = (uint32_t)ci.startindex_with_ghost;
+                    pl.cell[c].N = ci.count;
+                    n += ci.count;
+                } else {
+                    pl.cell[c].start = 0;
+                    pl.cell[c].N = 0;
+                }
+            }
+            out.src_plan[u] = pl;
+            out.src_count[u] = n;
+            nb = nblocks(n);
+        }
+        int tb;
+        (void)block_excl_scan(nb, &tb);
+        if (threadIdx.x == 0) out.part[2 * GP + g] = tb;
+    }
+}
+
+// The work items of the workgroup's pencils, taken in the host list's order
+// (thread t: pencil (j, k) = (t % jw, t / jw), so neighbours in y -- which
+// share 2R of their 2R+1 source pencils -- land next to each other in a
+// bucket).  Each item run gets its place in two steps: an offset within the
+// workgroup's share of its bucket (LDS atomics), then the share's base (one
+// global atomic per workgroup and bucket).  boff: the buckets' offsets in
+// each list.  Every thread of the workgroup must call it.
+__device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_t (*boff)[kPlanBuckets]) {
+    __shared__ int32_t lc[3 * kPlanBuckets];
+    const int P = in.jw * in.cpd;
+    for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT) lc[i] = 0;
     __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {  // inclusive scan of the run sums
-        const int64_t v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    const int t = blockIdx.x * kT + threadIdx.x;
+    const int s = t < P ? (t % in.jw) * in.cpd + t / in.jw : 0;
+    const int n = t < P ? out.sink_count[s] : 0;
+    const uint64_t nsrc = n > 0 ? sink_pencil_sources(in, s / in.cpd, s % in.cpd) : 0;
+    int bk[3][3], lo[3][3];  // per tile: bucket and local offset of the full, one-wave and tail runs
+    for (int ti = 0; ti < 3; ti++) {
+        const ItemSplit sp(n, 1 << ti);
+        const bool on = out.items[ti] != nullptr && n > 0;
+        bk[ti][0] = cost_bucket((uint64_t)sp.tile * nsrc);
+        bk[ti][1] = cost_bucket((uint64_t)kB * nsrc);
+        bk[ti][2] = cost_bucket((uint64_t)sp.s * nsrc);
+        int32_t *h = lc + ti * kPlanBuckets;
+        lo[ti][0] = on && sp.full ? atomicAdd(h + bk[ti][0], sp.full) : 0;
+        lo[ti][1] = on && sp.n2 ? atomicAdd(h + bk[ti][1], sp.n2) : 0;
+        lo[ti][2] = on && sp.tail ? atomicAdd(h + bk[ti][2], 1) : 0;
     }
-    int64_t run = part[t] - s;
-    for (int i = b; i < e; i++) {
-        const int32_t x = in[i];
-        out[i] = (int32_t)run;
-        run += x;
-    }
-}
-
-// Sink pencils: SinkPlan (SinkPencilPlan::load, PencilPlan.cc:95-138),
-// SinkSetCount, padded block and co-add chunk counts.
-__global__ __launch_bounds__(256) void k_plan_b_sinks(GPlanIn in, GPlanOut out) {
-    const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const ScratchMap m = scratch_map(P, Q);
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P) return;
-    const int j = s / in.cpd, k = s % in.cpd;
-    const size_t row = (size_t)(j + in.j_low) * in.cpd;
-    nf_sink_plan pl;
-    int n = 0;
-    for (int c = 0; c < NF_MAX_WIDTH; c++) {
-        if (c < in.W) {
-            const nf_cellinfo &ci = in.ci[in.R][row + wrapi(k + c, in.cpd)];
-            pl.cell[c].start = (uint32_t)ci.startindex;
-            pl.cell[c].N = ci.count;
-            n += pl.cell[c].N;
-        } else {
-            pl.cell[c].start = 0;
-            pl.cell[c].N = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT)
+        if (lc[i]) lc[i] = atomicAdd(out.cursor + i, lc[i]);
+    __syncthreads();
+    if (n == 0) return;
+    const int sic4 = 4 * out.sic;
+    for (int ti = 0; ti < 3; ti++) {
+        WorkItem *items = out.items[ti];
+        if (!items) continue;
+        const ItemSplit sp(n, 1 << ti);
+        const int32_t *base = lc + ti * kPlanBuckets;
+        if (sp.full) {
+            const int p = boff[ti][bk[ti][0]] + base[bk[ti][0]] + lo[ti][0];
+            for (int i = 0; i < sp.full; i++) items[p + i] = WorkItem{s, i * sp.tile, sp.tile, 0 + sic4};
         }
-    }
-    // The cell that sits at z = 0 after the periodic wrap (PencilPlan.cc:123-128):
-    // k + c = cpd, if that is within the pencil.
-    const int wc = (in.cpd - k) % in.cpd;
-    pl.wrapcell = wc < in.W ? wc : 0;
-    out.sink_plan[s] = pl;
-    out.sink_count[s] = n;
-    out.scratch[m.sblk + s] = nblocks(n);
-    out.scratch[m.cblk + s] = nblocks(pl.cell[0].N);
-}
-
-// Source pencils: SourcePlan (SourcePencilPlan::load, PencilPlan.cc:260-279),
-// SourceSetCount, padded block counts.
-__global__ __launch_bounds__(256) void k_plan_b_sources(GPlanIn in, GPlanOut out) {
-    const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const ScratchMap m = scratch_map(P, Q);
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= Q) return;
-    const int cell = source_cell(in, u / in.cpd, u % in.cpd);
-    nf_source_plan pl;
-    int n = 0;
-    for (int c = 0; c < NF_MAX_WIDTH; c++) {
-        if (c < in.W) {
-            const nf_cellinfo &ci = in.ci[c][cell];
-            pl.cell[c].start = (uint32_t)ci.startindex_with_ghost;
-            pl.cell[c].N = ci.count;
-            n += pl.cell[c].N;
-        } else {
-            pl.cell[c].start = 0;
-            pl.cell[c].N = 0;
+        const int f = sp.full * sp.tile;
+        if (sp.n2) {
+            const int p = boff[ti][bk[ti][1]] + base[bk[ti][1]] + lo[ti][1];
+            for (int i = 0; i < sp.n2; i++) items[p + i] = WorkItem{s, f + i * kB, i < sp.q ? kB : sp.s, 2 + sic4};
         }
+        if (sp.tail) items[boff[ti][bk[ti][2]] + base[bk[ti][2]] + lo[ti][2]] = WorkItem{s, f + sp.q * kB, sp.s, 1 + sic4};
     }
-    out.src_plan[u] = pl;
-    out.src_count[u] = n;
-    out.scratch[m.qblk + u] = nblocks(n);
 }
 
-// After the scans.  Sink pencils: SinkSetStart, SinkSetIdMax, the block
-// parents, the interaction list and y offsets (PencilOnPencil.cc:371-395),
-// the co-add chunks.
-__global__ __launch_bounds__(256) void k_plan_b_sinks2(GPlanIn in, GPlanOut out) {
+// Same workgroup layout as k_plan_b1.
+__global__ __launch_bounds__(kT) void k_plan_b2(GPlanIn in, GPlanOut out) {
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const ScratchMap m = scratch_map(P, Q);
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P) return;
-    const int j = s / in.cpd, k = s % in.cpd;
-    const int n = out.sink_count[s];
-    const int b0 = out.scratch[m.sscan + s];
-    out.sink_start[s] = b0 * kB;
-    out.idmax[s] = (uint32_t)(b0 * kB + n);
-    for (int q = 0, nb = nblocks(n); q < nb; q++) out.parent[b0 + q] = s;
-    const float cellsize = (float)(1.0 / (double)in.cpd);
-    for (int y = 0; y < in.W; y++) {
-        out.ilist[in.W * s + y] = (j + y) * in.cpd + k;
-        out.yoff[in.W * s + y] = (float)(y - in.R) * cellsize;
+    const int GP = ngroups(P);
+    if ((int)blockIdx.x < GP) {
+        // Bucket offsets of the wanted work lists: exclusive scan of the
+        // histogram in LDS (2 buckets per thread).
+        __shared__ int32_t boff[3][kPlanBuckets];
+        for (int ti = 0; ti < 3; ti++) {
+            if (!out.items[ti]) continue;
+            const int32_t *h = out.hist + ti * kPlanBuckets;
+            const int a = h[2 * threadIdx.x], b = h[2 * threadIdx.x + 1];
+            int total;
+            const int e = block_excl_scan(a + b, &total);
+            boff[ti][2 * threadIdx.x] = e;
+            boff[ti][2 * threadIdx.x + 1] = e + a;
+        }
+        const int s = blockIdx.x * kT + threadIdx.x;
+        const int n = s < P ? out.sink_count[s] : 0;
+        const int n0 = s < P ? out.sink_plan[s].cell[0].N : 0;
+        int tb, tc;
+        const int b0 = block_sum_prefix(out.part, blockIdx.x) + block_excl_scan(nblocks(n), &tb);
+        const int c0 = block_sum_prefix(out.part + GP, blockIdx.x) + block_excl_scan(nblocks(n0), &tc);
+        place_items(in, out, boff);
+        if (s >= P) return;
+        const int j = s / in.cpd, k = s % in.cpd;
+        out.sink_start[s] = b0 * kB;
+        out.idmax[s] = (uint32_t)(b0 * kB + n);
+        for (int q = 0, nb = nblocks(n); q < nb; q++) out.parent[b0 + q] = s;
+        const float cellsize = (float)(1.0 / (double)in.cpd);
+        for (int y = 0; y < in.W; y++) {
+            out.ilist[in.W * s + y] = (j + y) * in.cpd + k;
+            out.yoff[in.W * s + y] = (float)(y - in.R) * cellsize;
+        }
+        for (int f = 0, q = c0; f < n0; f += kB, q++) {
+            out.chunks[2 * q] = s;
+            out.chunks[2 * q + 1] = f;
+        }
+    } else {
+        const int g = blockIdx.x - GP;
+        const int u = g * kT + threadIdx.x;
+        const int n = u < Q ? out.src_count[u] : 0;
+        int tb;
+        const int b0 = block_sum_prefix(out.part + 2 * GP, g) + block_excl_scan(nblocks(n), &tb);
+        if (u >= Q) return;
+        out.src_start[u] = b0 * kB;
+        for (int q = 0, nb = nblocks(n); q < nb; q++) out.src_parent[b0 + q] = u;
     }
-    const int n0 = out.sink_plan[s].cell[0].N;
-    for (int f = 0, q = out.scratch[m.cscan + s]; f < n0; f += kB, q++) {
-        out.chunks[2 * q] = s;
-        out.chunks[2 * q + 1] = f;
-    }
 }
-
-// Source pencils: SourceSetStart and the blocks' parent pencils.
-__global__ __launch_bounds__(256) void k_plan_b_sources2(GPlanIn in, GPlanOut out) {
-    const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const ScratchMap m = scratch_map(P, Q);
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= Q) return;
-    const int b0 = out.scratch[m.qscan + u];
-    out.src_start[u] = b0 * kB;
-    for (int q = 0, nb = nblocks(out.src_count[u]); q < nb; q++) out.src_parent[b0 + q] = u;
-}
-
-// ---- Work list -------------------------------------------------------------
-
-// Thread t takes sink pencil (j, k) = (t % jw, t / jw): neighbours in y,
-// which share 2R of their 2R+1 source pencils, sit on adjacent lanes and
-// land close together in a bucket.
-__device__ __forceinline__ int item_pencil(const GPlanIn &in, int t) { return (t % in.jw) * in.cpd + t / in.jw; }
-
-__device__ __forceinline__ uint64_t pencil_sources(const GPlanIn &in, const GPlanOut &out, int s) {
-    const int j = s / in.cpd, k = s % in.cpd;
-    uint64_t nsrc = 0;
-    for (int y = 0; y < in.W; y++) nsrc += (uint64_t)out.src_count[(j + y) * in.cpd + k];
-    return nsrc;
-}
-
-__global__ __launch_bounds__(256) void k_items_hist(GPlanIn in, GPlanOut out, int S, int32_t *hist) {
-    const int P = in.jw * in.cpd;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P) return;
-    const int s = item_pencil(in, t);
-    const uint64_t nsrc = pencil_sources(in, out, s);
-    for_items(out.sink_count[s], S, [&](int, int m, int kind) {
-        atomicAdd(hist + cost_bucket((uint64_t)(kind == 2 ? kB : m) * nsrc), 1);
-    });
-}
-
-__global__ __launch_bounds__(256) void k_items_place(GPlanIn in, GPlanOut out, int S, WorkItem *items,
-                                                     const int32_t *offset, int32_t *cursor) {
-    const int P = in.jw * in.cpd;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P) return;
-    const int s = item_pencil(in, t);
-    const uint64_t nsrc = pencil_sources(in, out, s);
-    for_items(out.sink_count[s], S, [&](int first, int m, int kind) {
-        const int b = cost_bucket((uint64_t)(kind == 2 ? kB : m) * nsrc);
-        items[offset[b] + atomicAdd(cursor + b, 1)] = WorkItem{s, first, m, kind + 4 * out.sic};
-    });
-}
-
-__global__ __launch_bounds__(256) void k_zero_i32(int32_t *p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
-inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 
-size_t gpu_plan_scratch_ints(int nsink_sets, int nsrc_sets) {
-    return (size_t)scratch_map(nsink_sets, nsrc_sets).hist + 3 * (size_t)kPlanBuckets;
+size_t gpu_plan_part_ints(int nsink_sets, int nsrc_sets) {
+    return 2 * (size_t)ngroups(nsink_sets) + (size_t)ngroups(nsrc_sets);
 }
 
-int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *scratch, void *stream) {
+int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *shared, void *stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    hipLaunchKernelGGL(k_zero_totals, dim3(1), dim3(64), 0, st, tot);
-    hipLaunchKernelGGL(k_plan_a_counts, grid_for(P + Q), dim3(256), 0, st, in, tot, scratch);
-    hipLaunchKernelGGL(k_plan_a_direct, grid_for(P), dim3(256), 0, st, in, tot, scratch);
+    hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(kT), 0, st, tot, shared, shared ? kPlanSharedInts : 0);
+    hipLaunchKernelGGL(k_plan_a, dim3(ngroups(P + Q)), dim3(kT), 0, st, in, tot);
     return (int)hipGetLastError();
 }
 
-int launch_gpu_plan_b(const GPlanIn &in, const GPlanOut &out, void *stream) {
-    hipStream_t st = static_cast<hipStream_t>(stream);
+int launch_gpu_plan_b1(const GPlanIn &in, const GPlanOut &out, void *stream) {
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const ScratchMap m = scratch_map(P, Q);
-    int32_t *sc = out.scratch;
-    hipLaunchKernelGGL(k_plan_b_sinks, grid_for(P), dim3(256), 0, st, in, out);
-    hipLaunchKernelGGL(k_plan_b_sources, grid_for(Q), dim3(256), 0, st, in, out);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(kScanThreads), 0, st, sc + m.sblk, sc + m.sscan, P);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(kScanThreads), 0, st, sc + m.qblk, sc + m.qscan, Q);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(kScanThreads), 0, st, sc + m.cblk, sc + m.cscan, P);
-    hipLaunchKernelGGL(k_plan_b_sinks2, grid_for(P), dim3(256), 0, st, in, out);
-    hipLaunchKernelGGL(k_plan_b_sources2, grid_for(Q), dim3(256), 0, st, in, out);
+    hipLaunchKernelGGL(k_plan_b1, dim3(ngroups(P) + ngroups(Q)), dim3(kT), 0, static_cast<hipStream_t>(stream), in,
+                       out);
     return (int)hipGetLastError();
 }
 
-int launch_gpu_plan_items(const GPlanIn &in, const GPlanOut &out, int S, WorkItem *items, void *stream) {
-    hipStream_t st = static_cast<hipStream_t>(stream);
+int launch_gpu_plan_b2(const GPlanIn &in, const GPlanOut &out, void *stream) {
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    int32_t *hist = out.scratch + scratch_map(P, Q).hist, *offset = hist + kPlanBuckets,
-            *cursor = offset + kPlanBuckets;
-    hipLaunchKernelGGL(k_zero_i32, grid_for(3 * kPlanBuckets), dim3(256), 0, st, hist, 3 * kPlanBuckets);
-    hipLaunchKernelGGL(k_items_hist, grid_for(P), dim3(256), 0, st, in, out, S, hist);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(kScanThreads), 0, st, hist, offset, kPlanBuckets);
-    hipLaunchKernelGGL(k_items_place, grid_for(P), dim3(256), 0, st, in, out, S, items, offset, cursor);
+    hipLaunchKernelGGL(k_plan_b2, dim3(ngroups(P) + ngroups(Q)), dim3(kT), 0, static_cast<hipStream_t>(stream), in,
+                       out);
     return (int)hipGetLastError();
 }
 

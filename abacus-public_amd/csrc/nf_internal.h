@@ -134,26 +134,30 @@ struct GPlanTotals {
     uint64_t occ[4];    // sinks in pencils of <64, 64-255, 256-1023, >=1024 sinks
     uint64_t bad;       // planned cells with a negative count or outside their slab
 };
-// Where phase B writes (the plan buffer at the SIC's PlanLayout offsets),
-// and its scratch of gpu_plan_scratch_ints() int32 (per-pencil counts and
-// their prefix sums, the work-list histogram).
+// Where phase B writes (the plan buffer at the SIC's PlanLayout offsets).
+// items[t]: the work list for sinks_per_lane 1 << t, or null if not wanted;
+// hist and cursor (kPlanBuckets int32 per tile each, zeroed by phase A of the
+// group's first SIC) are shared by the SICs of a launch group, whose lists
+// are merged; part holds gpu_plan_part_ints() int32 of this SIC's block sums.
 struct GPlanOut {
     int32_t *sink_start, *sink_count, *parent, *src_start, *src_count, *src_parent, *ilist, *chunks;
     uint32_t *idmax;
     nf_sink_plan *sink_plan;
     nf_source_plan *src_plan;
     float *yoff;
+    WorkItem *items[3];
+    int32_t *hist, *cursor, *part;
     int32_t sic;  // index of the SIC in its launch group (the work items' kind_sic)
-    int32_t *scratch;
 };
-constexpr int kPlanBuckets = 512;  // 1/8-octave cost buckets of a device-built work list
-size_t gpu_plan_scratch_ints(int nsink_sets, int nsrc_sets);
-// Phase A: the totals (sizes) of the SIC; the scratch must be the one phase
-// B gets.  Phase B: its arrays.  Items: the work list for sinks_per_lane S
-// (after phase B), longest first in 1/8-octave cost buckets.
-int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *scratch, void *stream);
-int launch_gpu_plan_b(const GPlanIn &in, const GPlanOut &out, void *stream);
-int launch_gpu_plan_items(const GPlanIn &in, const GPlanOut &out, int S, WorkItem *items, void *stream);
+constexpr int kPlanBuckets = 512;                   // 1/8-octave cost buckets of a device-built work list
+constexpr int kPlanSharedInts = 2 * 3 * kPlanBuckets;  // hist + cursor
+size_t gpu_plan_part_ints(int nsink_sets, int nsrc_sets);
+// Phase A: the SIC's totals (sizes); `shared` (hist + cursor, or null) is
+// zeroed first.  Phase B, after the host laid the plan out: B1 for every SIC
+// of the group, then B2 for every SIC.
+int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *shared, void *stream);
+int launch_gpu_plan_b1(const GPlanIn &in, const GPlanOut &out, void *stream);
+int launch_gpu_plan_b2(const GPlanIn &in, const GPlanOut &out, void *stream);
 // The occupancy target (waves per SIMD) the batched kernel is built with for
 // a pair form and tile; the third template argument of k_batched.
 int batched_kernel_waves(int softening, int sinks_per_lane);

@@ -267,8 +267,8 @@ PlanLayout plan_layout(const nf_sic &s, size_t base, int32_t n_chunks) {
 }
 
 // Where the device planner writes a SIC's arrays: the plan buffer `base` at
-// the offsets of L.
-GPlanOut plan_out(char *base, const PlanLayout &L, int sic, int32_t *scratch) {
+// the offsets of L; no work lists (the caller names them).
+GPlanOut plan_out(char *base, const PlanLayout &L, int sic, int32_t *shared, int32_t *part) {
     GPlanOut o{};
     o.sink_start = reinterpret_cast<int32_t *>(base + L.o_sink_start);
     o.sink_count = reinterpret_cast<int32_t *>(base + L.o_sink_count);
@@ -282,8 +282,10 @@ GPlanOut plan_out(char *base, const PlanLayout &L, int sic, int32_t *scratch) {
     o.yoff = reinterpret_cast<float *>(base + L.o_yoff);
     o.src_parent = reinterpret_cast<int32_t *>(base + L.o_src_parent);
     o.chunks = reinterpret_cast<int32_t *>(base + L.o_chunks);
+    o.hist = shared;
+    o.cursor = shared ? shared + 3 * kPlanBuckets : nullptr;
+    o.part = part;
     o.sic = sic;
-    o.scratch = scratch;
     return o;
 }
 
@@ -367,8 +369,8 @@ void task_rows(const Task &t, uint64_t *r0, uint64_t *r1) {
 }
 
 // Device bytes a SIC adds to a group (plan + work + host-mode slabs).
-size_t task_bytes(const nf_sic &s, int path) {
-    size_t b = plan_layout(s, 0, count_coadd_chunks(s)).total + work_layout(s, path, 0).total + sizeof(SicView) + 256;
+size_t task_bytes(const nf_sic &s, int path, int32_t n_chunks) {
+    size_t b = plan_layout(s, 0, n_chunks).total + work_layout(s, path, 0).total + sizeof(SicView) + 256;
     b += ((size_t)s.NSinkBlocks + 2 * (size_t)s.NSinkSets) * sizeof(WorkItem);  // at most one item per block + tails
     if (s.mem == NF_MEM_HOST) {
         for (int c = 0; c < s.nfwidth; c++) b += align_up(12 * (size_t)s.Nslab[c], 256);
@@ -376,6 +378,33 @@ size_t task_bytes(const nf_sic &s, int path) {
     }
     return b;
 }
+
+// A device-planned SIC before phase A: upper bounds of its sizes from its
+// slab sizes (every pencil adds at most one partly filled block; a source
+// row that wraps round the box appears twice), and an estimate of its pair
+// interactions for a uniform box, (2R+1)^2 n_sink n_window / cpd^2.  For
+// grouping decisions only.
+size_t dp_task_bytes(const nf_sic &s0, int path) {
+    nf_sic s = s0;
+    const uint64_t W = (uint64_t)s.nfwidth, cap = (uint64_t)INT32_MAX / NF_BLOCK_SIZE;
+    uint64_t window = 0;
+    for (int c = 0; c < s.nfwidth; c++) window += s.Nslab[c];
+    const uint64_t wraps = ((uint64_t)(s.j_width + s.nfwidth - 1) + s.cpd - 1) / (uint64_t)s.cpd;
+    s.NSinkBlocks = (int32_t)std::min<uint64_t>(cap, W * s.Nslab[s.nfradius] / NF_BLOCK_SIZE + s.NSinkSets);
+    s.NSourceBlocks = (int32_t)std::min<uint64_t>(cap, wraps * window / NF_BLOCK_SIZE + s.NSourceSets);
+    const uint64_t chunks = std::min<uint64_t>(cap, s.Nslab[s.nfradius] / NF_BLOCK_SIZE + s.NSinkSets);
+    return task_bytes(s, path, (int32_t)chunks);
+}
+uint64_t dp_interactions_estimate(const nf_sic &s) {
+    double window = 0.0;
+    for (int c = 0; c < s.nfwidth; c++) window += (double)s.Nslab[c];
+    const double W = s.nfwidth, cpd = s.cpd;
+    return (uint64_t)(W * W * (double)s.Nslab[s.nfradius] * window / (cpd * cpd) * s.j_width / cpd);
+}
+size_t queued_bytes(const Task &t, int path) {
+    return t.dp.on ? dp_task_bytes(t.sic, path) : task_bytes(t.sic, path, count_coadd_chunks(t.sic));
+}
+uint64_t queued_interactions(const Task &t) { return t.dp.on ? dp_interactions_estimate(t.sic) : t.sic.DirectTotal; }
 
 // The batched work list of one SIC: per sink pencil, full register tiles of
 // 64*S sinks, then the tail; cost = sinks x sources of the item.
@@ -674,41 +703,70 @@ struct SlotRelease {
     }
 };
 
-// Device planning, phase A, of a task from nf_submit_slab: its cellinfo
-// slabs go up through the slab cache (held in `cslot` until the group is
-// done), the planner's first kernels run on the worker's planning stream,
-// and the plan's sizes come back into t.sic.  Fills *in for phase B.
-int plan_phase_a(Runtime &rt, Worker &w, Task &t, std::vector<Runtime::Slot *> &cslot, GPlanIn *in,
-                 uint64_t *to_dev) {
-    nf_sic &s = t.sic;
-    std::memset(in, 0, sizeof(*in));
-    in->cpd = s.cpd;
-    in->R = s.nfradius;
-    in->W = s.nfwidth;
-    in->jw = s.j_width;
-    in->j_low = s.j_low;
-    const uint64_t cbytes = sizeof(nf_cellinfo) * (uint64_t)s.cpd * (uint64_t)s.cpd;
-    for (int c = 0; c < s.nfwidth; c++) {
-        Runtime::Slot *sl = nullptr;
-        bool up = false;
-        int rc = acquire_slab(rt, t.dp.cellinfo[c], cbytes, &sl, &up);
-        if (rc) return rc;
-        cslot.push_back(sl);
-        if (up) *to_dev += cbytes;
-        in->ci[c] = static_cast<const nf_cellinfo *>(sl->buf.p);
-        in->nslab[c] = s.Nslab[c];
-        NF_HIP(hipStreamWaitEvent(w.pstream, sl->ready, 0));
+// The device planner's scratch for a group of SICs: each SIC's totals
+// (256 bytes apiece, read back together), the work-list histogram and
+// cursors the group shares, each SIC's block sums.
+struct GPlanScratch {
+    size_t o_shared = 0, total = 0;
+    std::vector<size_t> o_part;
+};
+GPlanScratch gplan_scratch(const std::vector<Task *> &g) {
+    static_assert(sizeof(GPlanTotals) <= 256, "totals slot");
+    GPlanScratch L;
+    size_t o = 256 * g.size();
+    L.o_shared = o;
+    o = align_up(o + sizeof(int32_t) * kPlanSharedInts, 256);
+    for (Task *t : g) {
+        L.o_part.push_back(o);
+        o = align_up(o + sizeof(int32_t) * gpu_plan_part_ints(t->sic.NSinkSets, t->sic.NSourceSets), 256);
     }
-    int rc = w.gscratch.ensure(256 + sizeof(int32_t) * gpu_plan_scratch_ints(s.NSinkSets, s.NSourceSets));
-    if (rc || (rc = w.pin_tot.ensure(sizeof(GPlanTotals)))) return rc;
-    GPlanTotals *tot = static_cast<GPlanTotals *>(w.gscratch.p);
-    int32_t *scratch = reinterpret_cast<int32_t *>(static_cast<char *>(w.gscratch.p) + 256);
-    const int e = launch_gpu_plan_a(*in, tot, scratch, w.pstream);
-    if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
-    NF_HIP(hipMemcpyAsync(w.pin_tot.p, tot, sizeof(GPlanTotals), hipMemcpyDeviceToHost, w.pstream));
+    L.total = o;
+    return L;
+}
+
+// Device planning, phase A, of a group of tasks from nf_submit_slab: their
+// cellinfo slabs go up through the slab cache (held in `cslot` until the
+// group is done), the counting kernels run on the worker's planning stream,
+// and the plans' sizes come back (one copy, one wait) into each t->sic.
+// Fills in[i] for phase B.
+int plan_phase_a(Runtime &rt, Worker &w, std::vector<Task *> &g, std::vector<Runtime::Slot *> &cslot,
+                 std::vector<GPlanIn> &in, const GPlanScratch &gs, uint64_t *to_dev) {
+    const size_t n = g.size();
+    int rc = w.gscratch.ensure(gs.total);
+    if (rc || (rc = w.pin_tot.ensure(256 * n))) return rc;
+    char *sc = static_cast<char *>(w.gscratch.p);
+    in.assign(n, GPlanIn{});
+    for (size_t i = 0; i < n; i++) {
+        Task &t = *g[i];
+        const nf_sic &s = t.sic;
+        GPlanIn &gi = in[i];
+        gi.cpd = s.cpd;
+        gi.R = s.nfradius;
+        gi.W = s.nfwidth;
+        gi.jw = s.j_width;
+        gi.j_low = s.j_low;
+        const uint64_t cbytes = sizeof(nf_cellinfo) * (uint64_t)s.cpd * (uint64_t)s.cpd;
+        for (int c = 0; c < s.nfwidth; c++) {
+            Runtime::Slot *sl = nullptr;
+            bool up = false;
+            if ((rc = acquire_slab(rt, t.dp.cellinfo[c], cbytes, &sl, &up))) return rc;
+            cslot.push_back(sl);
+            if (up) *to_dev += cbytes;
+            gi.ci[c] = static_cast<const nf_cellinfo *>(sl->buf.p);
+            gi.nslab[c] = s.Nslab[c];
+            NF_HIP(hipStreamWaitEvent(w.pstream, sl->ready, 0));
+        }
+        const int e = launch_gpu_plan_a(gi, reinterpret_cast<GPlanTotals *>(sc + 256 * i),
+                                        i == 0 ? reinterpret_cast<int32_t *>(sc + gs.o_shared) : nullptr, w.pstream);
+        if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
+    }
+    NF_HIP(hipMemcpyAsync(w.pin_tot.p, sc, 256 * n, hipMemcpyDeviceToHost, w.pstream));
     NF_HIP(hipStreamSynchronize(w.pstream));
-    std::memcpy(&t.dp.tot, w.pin_tot.p, sizeof(GPlanTotals));
-    return apply_totals(s, t.dp.tot);
+    for (size_t i = 0; i < n; i++) {
+        std::memcpy(&g[i]->dp.tot, static_cast<char *>(w.pin_tot.p) + 256 * i, sizeof(GPlanTotals));
+        if ((rc = apply_totals(g[i]->sic, g[i]->dp.tot))) return rc;
+    }
+    return NF_OK;
 }
 
 // Execute a group of compatible SICs on worker w: one plan upload, a gather
@@ -720,16 +778,19 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     const int path = rt.params.path;
     TileShape ts = rt.shape;
     const int n = (int)g.size();
-    // A task from nf_submit_slab (always alone in its group) is planned on
-    // the device: phase A first, since everything below needs its sizes.
+    // Tasks from nf_submit_slab (grouped only with each other) are planned
+    // on the device: phase A first, since everything below needs the sizes.
     const bool dplan = g[0]->dp.on;
-    GPlanIn gin{};
+    std::vector<GPlanIn> gin;
+    GPlanScratch gs;
     std::vector<Runtime::Slot *> cslot;
     SlotRelease cell_release{rt, cslot};
     uint64_t cell_bytes = 0;
     if (dplan) {
-        if (n != 1) return fail(NF_ERR_STATE, "a device-planned SIC must run alone");
-        const int rc = plan_phase_a(rt, w, *g[0], cslot, &gin, &cell_bytes);
+        for (Task *t : g)
+            if (!t->dp.on) return fail(NF_ERR_STATE, "device-planned and host-planned SICs in one group");
+        gs = gplan_scratch(g);
+        const int rc = plan_phase_a(rt, w, g, cslot, gin, gs, &cell_bytes);
         if (rc) return rc;
     }
     const auto t_plan = clk::now();
@@ -791,7 +852,10 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     std::vector<size_t> nitems;
     for (int S : Svals) {
         itemsets.push_back(dplan ? std::vector<WorkItem>() : make_items(S));
-        nitems.push_back(dplan ? (size_t)g[0]->dp.tot.items[S == 1 ? 0 : S == 2 ? 1 : 2] : itemsets.back().size());
+        size_t m = itemsets.back().size();
+        if (dplan)
+            for (Task *t : g) m += (size_t)t->dp.tot.items[S == 1 ? 0 : S == 2 ? 1 : 2];
+        nitems.push_back(m);
     }
     auto items_of = [&](int S) -> int {
         for (size_t k = 0; k < Svals.size(); k++)
@@ -849,7 +913,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         return fail(NF_ERR_CAPACITY, "SIC group needs " + std::to_string(plan_total + work_total + slab_total) +
                                          " device bytes, buffer holds " + std::to_string(rt.max_buffer_bytes));
     if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) ||
-        (rc = w.pin_plan.ensure(dplan ? sizeof(SicView) : plan_total)))
+        (rc = w.pin_plan.ensure(dplan ? sizeof(SicView) * (size_t)n : plan_total)))
         return rc;
     if (s0.mem == NF_MEM_HOST && (rc = w.slabs.ensure(slab_total))) return rc;
 
@@ -982,12 +1046,20 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     // plan buffer is free).  A device-planned SIC's image is its view only;
     // phase B and the work lists fill the rest on the planning stream.
     if (dplan) {
-        NF_HIP(hipMemcpyAsync(w.plan.p, hp, sizeof(SicView), hipMemcpyHostToDevice, w.pstream));
-        const GPlanOut go = plan_out(pd, PL[0], 0,
-                                     reinterpret_cast<int32_t *>(static_cast<char *>(w.gscratch.p) + 256));
-        int e = launch_gpu_plan_b(gin, go, w.pstream);
-        for (size_t k = 0; k < Svals.size() && !e; k++)
-            e = launch_gpu_plan_items(gin, go, Svals[k], reinterpret_cast<WorkItem *>(pd + o_itemset[k]), w.pstream);
+        NF_HIP(hipMemcpyAsync(w.plan.p, hp, sizeof(SicView) * (size_t)n, hipMemcpyHostToDevice, w.pstream));
+        char *sc = static_cast<char *>(w.gscratch.p);
+        int32_t *shared = reinterpret_cast<int32_t *>(sc + gs.o_shared);
+        std::vector<GPlanOut> go(n);
+        for (int i = 0; i < n; i++) {
+            go[i] = plan_out(pd, PL[i], i, shared, reinterpret_cast<int32_t *>(sc + gs.o_part[i]));
+            for (size_t k = 0; k < Svals.size(); k++) {
+                const int ti = Svals[k] == 1 ? 0 : Svals[k] == 2 ? 1 : 2;
+                go[i].items[ti] = reinterpret_cast<WorkItem *>(pd + o_itemset[k]);
+            }
+        }
+        int e = 0;
+        for (int i = 0; i < n && !e; i++) e = launch_gpu_plan_b1(gin[i], go[i], w.pstream);
+        for (int i = 0; i < n && !e; i++) e = launch_gpu_plan_b2(gin[i], go[i], w.pstream);
         if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
         NF_HIP(hipEventRecord(w.gplan_ready, w.pstream));
     } else {
@@ -1028,7 +1100,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     hipStream_t stream = rt.shared ? rt.shared : w.stream;
     std::unique_lock<std::mutex> enqueue_lock(rt.enqueue, std::defer_lock);
     if (rt.shared) enqueue_lock.lock();
-    uint64_t to_dev = (dplan ? sizeof(SicView) : plan_total) + far_bytes + cell_bytes, from_dev = 0;
+    uint64_t to_dev = (dplan ? sizeof(SicView) * (size_t)n : plan_total) + far_bytes + cell_bytes, from_dev = 0;
     // With graphs, the sequence below is captured and launched as one graph.
     const bool use_graph = rt.params.use_graphs && !trial;
     struct CaptureGuard {  // ends an open capture on every early return
@@ -1232,22 +1304,23 @@ void worker_main(Runtime *rt, Worker *w) {
             if (rt->queue.empty()) return;  // stopping and drained
             g.push_back(rt->queue.front());
             rt->queue.pop_front();
-            // A device-planned SIC runs alone (its sizes are not known yet).
-            const bool solo = g[0]->dp.on;
-            size_t bytes = solo ? 0 : task_bytes(g[0]->sic, rt->params.path);
+            // Device-planned SICs are grouped with each other only, on
+            // bounds of their sizes (the exact ones come with phase A).
+            const int path = rt->params.path;
+            size_t bytes = queued_bytes(*g[0], path);
             // After the queue ran dry the GPU is idle: start it on a single SIC
             // instead of waiting for a whole group to be prepared.
-            const int limit = solo || (!rt->warm && rt->inflight == 0) ? 1 : rt->max_group;
+            const int limit = !rt->warm && rt->inflight == 0 ? 1 : rt->max_group;
             rt->warm = true;
             rt->inflight++;
-            uint64_t work = g[0]->sic.DirectTotal;
-            while (rt->params.path == NF_PATH_BATCHED && (int)g.size() < limit && work < rt->group_interactions &&
-                   !rt->queue.empty() && !rt->queue.front()->dp.on &&
+            uint64_t work = queued_interactions(*g[0]);
+            while (path == NF_PATH_BATCHED && (int)g.size() < limit && work < rt->group_interactions &&
+                   !rt->queue.empty() && rt->queue.front()->dp.on == g[0]->dp.on &&
                    compatible(g[0]->sic, rt->queue.front()->sic)) {
-                const size_t b = task_bytes(rt->queue.front()->sic, rt->params.path);
+                const size_t b = queued_bytes(*rt->queue.front(), path);
                 if (bytes + b > rt->max_buffer_bytes) break;
                 bytes += b;
-                work += rt->queue.front()->sic.DirectTotal;
+                work += queued_interactions(*rt->queue.front());
                 g.push_back(rt->queue.front());
                 rt->queue.pop_front();
             }
@@ -1570,7 +1643,7 @@ extern "C" int nf_plan_sic_device(int32_t cpd, int32_t R, int32_t slab, int32_t 
     const size_t cbytes = sizeof(nf_cellinfo) * (size_t)cpd * (size_t)cpd;
     NF_HIP(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
     NF_HIP(hipMalloc(&b.cells, cbytes * W));
-    NF_HIP(hipMalloc(&b.scr, 256 + sizeof(int32_t) * gpu_plan_scratch_ints(s.NSinkSets, s.NSourceSets)));
+    NF_HIP(hipMalloc(&b.scr, 256 + sizeof(int32_t) * gpu_plan_part_ints(s.NSinkSets, s.NSourceSets)));
     GPlanIn in{};
     in.cpd = cpd;
     in.R = R;
@@ -1583,8 +1656,8 @@ extern "C" int nf_plan_sic_device(int32_t cpd, int32_t R, int32_t slab, int32_t 
         NF_HIP(hipMemcpyAsync(const_cast<nf_cellinfo *>(in.ci[c]), cellinfo[c], cbytes, hipMemcpyHostToDevice, b.st));
     }
     GPlanTotals *tot_d = static_cast<GPlanTotals *>(b.scr);
-    int32_t *scratch = reinterpret_cast<int32_t *>(static_cast<char *>(b.scr) + 256);
-    int e = launch_gpu_plan_a(in, tot_d, scratch, b.st);
+    int32_t *part = reinterpret_cast<int32_t *>(static_cast<char *>(b.scr) + 256);
+    int e = launch_gpu_plan_a(in, tot_d, nullptr, b.st);
     if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
     GPlanTotals tot;
     NF_HIP(hipMemcpyAsync(&tot, tot_d, sizeof(tot), hipMemcpyDeviceToHost, b.st));
@@ -1592,7 +1665,9 @@ extern "C" int nf_plan_sic_device(int32_t cpd, int32_t R, int32_t slab, int32_t 
     if ((rc = apply_totals(s, tot))) return rc;
     const PlanLayout L = plan_layout(s, 0, (int32_t)tot.chunks);
     NF_HIP(hipMalloc(&b.plan, L.total));
-    e = launch_gpu_plan_b(in, plan_out(static_cast<char *>(b.plan), L, 0, scratch), b.st);
+    const GPlanOut go = plan_out(static_cast<char *>(b.plan), L, 0, nullptr, part);
+    e = launch_gpu_plan_b1(in, go, b.st);
+    if (!e) e = launch_gpu_plan_b2(in, go, b.st);
     if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
 
     // The arrays, carved from the arena as nf_plan_sic carves them.

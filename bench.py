@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--graphs", type=int, default=None,
                     help="capture each group's kernels in a HIP graph (default: on for slab streams)")
     ap.add_argument("--paths", default="plain,batched")
+    ap.add_argument("--planner", default="host", choices=["host", "device"],
+                    help="host: SICs planned by nf_plan_sic before the timed steps; device: each step submits "
+                         "the slabs with nf_submit_slab, which plans them on the GPU inside the timed region")
     ap.add_argument("--bpd", type=int, default=4, help="worker streams (DirectBPD)")
     ap.add_argument("--sinks-per-lane", type=int, default=0, help="0: the autotuner picks per occupancy histogram")
     ap.add_argument("--max-group", type=int, default=8, help="SICs per persistent launch (batched path)")
@@ -124,16 +127,20 @@ def main():
                                  use_graphs=args.graphs or 0) as drv:
             tp = time.time()
             sics = []
+            ptrs_of = {x: [pos_d[(x - R + c) % cpd].ptr for c in range(2 * R + 1)] for x in range(cpd)}
             for x in range(cpd):
-                ptrs = [pos_d[(x - R + c) % cpd].ptr for c in range(2 * R + 1)]
-                sics.append(drv.plan(box, x, pos_ptrs=ptrs, acc_ptr=acc_d[x].ptr, mem=nfm.NF_MEM_DEVICE))
+                sics.append(drv.plan(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr, mem=nfm.NF_MEM_DEVICE))
             plan_s = time.time() - tp
             direct_total = sum(int(s.desc.DirectTotal) for s in sics)
             padded_total = sum(int(s.desc.PaddedDirectTotal) for s in sics)
             log(f"[bench] {path}: planned {cpd} SICs in {plan_s:.2f}s, DirectTotal={direct_total:.4e}")
 
             def step():
-                tasks = [drv.submit(s) for s in sics]
+                if args.planner == "device":
+                    tasks = [drv.submit_slab(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr,
+                                             mem=nfm.NF_MEM_DEVICE) for x in range(cpd)]
+                else:
+                    tasks = [drv.submit(s) for s in sics]
                 st = []
                 for t in tasks:
                     t.wait()
@@ -177,6 +184,7 @@ def main():
                 launches_per_step=launches / args.steps,
                 interactions_per_launch=float(per_slab_di.sum() / launches),
                 direct_total=direct_total, padded_direct_total=padded_total, plan_s=plan_s,
+                device_plan_s_per_step=float(sum(s["plan_s"] for s in kstats)) / args.steps,
                 kernels=shapes, n_sics=len(sics),
                 sic_sizes=dict(padded_sinks=float(np.mean([s.desc.PaddedSinkTotal for s in sics])),
                                padded_sources=float(np.mean([s.desc.PaddedSourceTotal for s in sics]))),
@@ -209,7 +217,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": desc, "N": box.N, "cpd": cpd, "nfradius": R, "path": best,
-                   "softening": "single_spline", "interactions_per_step": r["direct_total"]},
+                   "softening": "single_spline", "interactions_per_step": r["direct_total"],
+                   "planner": args.planner},
         "pct_fp32_peak": 100.0 * FLOP_PER_INTERACTION * r["interactions_per_s"] / (PEAK_FP32_TFLOPS * 1e12),
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": traffic,
@@ -417,7 +426,15 @@ def run_stream(args):
 
         def step():
             drv.stream_begin()
-            tasks = [drv.submit(s) for s in sics]
+            if args.planner == "device":
+                tasks = [drv.submit_slab(box, x, acc_ptr=acc[x].ctypes.data,
+                                         pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data
+                                                                            for c in range(2 * R + 1)],
+                                         mem=nfm.NF_MEM_HOST,
+                                         pos_layout=nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ,
+                                         kick=kicks.get(x)) for x in sinks]
+            else:
+                tasks = [drv.submit(s) for s in sics]
             st = []
             for t in tasks:
                 t.wait()
@@ -461,7 +478,8 @@ def run_stream(args):
                    "particles_in_window": int(sum(box.slab(x).n for x in range(-R, len(sinks) + R))),
                    "interactions_per_step": direct_total, "path": "batched", "graphs": graphs,
                    "softening": "single_spline", "mem": "host pinned" if args.pinned else "host pageable",
-                   "positions": "posstruct (AoS)" if args.aos else "PosXYZ", "fused_kick": bool(args.fuse_kick)},
+                   "positions": "posstruct (AoS)" if args.aos else "PosXYZ", "fused_kick": bool(args.fuse_kick),
+                   "planner": args.planner},
         "end_to_end": {
             "interactions_per_s": di / wall,
             "median_step_ms": 1e3 * float(np.median(walls)),
@@ -474,6 +492,7 @@ def run_stream(args):
             "slabs_reused_per_step": sum(x["slabs_reused"] for x in stats) / args.steps,
             "hidden_transfer_fraction": hidden,
             "plan_s": plan_s,
+            "device_plan_s_per_step": sum(x["plan_s"] for x in stats) / args.steps,
         },
         "autotune": [{k: e[k] for k in ("occupancy_quarters", "size_class", "sinks_per_lane", "uses")}
                      for e in table],

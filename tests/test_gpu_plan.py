@@ -103,7 +103,31 @@ def test_submit_slab_matches_submit(path, mem):
         assert st[i]["device_planned"] == 1 and st[i]["plan_s"] > 0
         assert st[i]["direct_total"] == sic.desc.DirectTotal
         assert st[i]["padded_direct_total"] == sic.desc.PaddedDirectTotal
-        assert st[i]["group_size"] == 1
+        assert 1 <= st[i]["group_size"] <= 8
+
+
+def test_device_planned_groups_match_single_launches():
+    """Device-planned SICs sharing one persistent launch (their work lists
+    merged on the device through one histogram) give bitwise the results of
+    one launch per SIC and of host-planned groups."""
+    box = small_uniform(cpd=9, ppc=20, seed=11)
+    eps, b2, _ = params(box)
+    out = {}
+    for planner, mg in (("device", 1), ("device", 8), ("host", 8)):
+        with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                                 max_buffer_bytes=2 << 30, sinks_per_lane=2, max_group=mg) as drv:
+            accs = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in range(box.cpd)}
+            for x in range(box.cpd):
+                drv.execute_slab(box, x, accs[x], nsplit=2, planner=planner)
+            sizes = []
+            for x in range(box.cpd):
+                sizes += [s["group_size"] for s in drv.finalize(x)]
+            out[(planner, mg)] = (accs, sizes)
+    assert max(out[("device", 1)][1]) == 1 and max(out[("device", 8)][1]) > 1
+    for x in range(box.cpd):
+        assert np.isfinite(out[("device", 8)][0][x]).all()
+        assert np.array_equal(out[("device", 1)][0][x], out[("device", 8)][0][x]), x
+        assert np.array_equal(out[("host", 8)][0][x], out[("device", 8)][0][x]), x
 
 
 def test_submit_slab_splits_ragged_and_autotuned():
@@ -140,9 +164,9 @@ def test_submit_slab_in_stream_session():
     eps, b2, _ = params(box)
     slabs = list(range(box.cpd))
     res = {}
-    for session in (False, True):
+    for session in (False, True):  # one SIC per launch: a group would share its slabs without a session
         with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=2,
-                                 max_buffer_bytes=2 << 30, sinks_per_lane=2) as drv:
+                                 max_buffer_bytes=2 << 30, sinks_per_lane=2, max_group=1) as drv:
             if session:
                 drv.stream_begin()
             accs = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in slabs}
