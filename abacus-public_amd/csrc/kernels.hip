@@ -273,6 +273,9 @@ __device__ __forceinline__ void partial_slots(const DevPlan &pl, int j, int z, i
 __device__ __forceinline__ float4 apply_kick(float4 a, const KickArgs &k, int64_t row) {
 #pragma clang fp contract(off)
     if (!k.on) return a;
+    if (k.near_out)  // the NearAcc record: FLOAT3p1 *= FLOAT scales xyz only
+        reinterpret_cast<float4 *>(k.near_out)[row] =
+            make_float4(a.x * k.inv_eps3, a.y * k.inv_eps3, a.z * k.inv_eps3, a.w);
     const float *f = k.far + 3 * (row - k.far_base);
     a.x = (a.x * k.inv_eps3 + f[0]) * k.rescale;
     a.y = (a.y * k.inv_eps3 + f[1]) * k.rescale;

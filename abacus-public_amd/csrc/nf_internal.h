@@ -69,9 +69,11 @@ struct DevSlabs {
 
 // The consumer's RescaleAndCoAddAcceleration, applied by the co-add kernels
 // to each sink's float4 (on == 0: off).  far[3 (row - far_base) + k] is the
-// far-field acceleration of slab row `row`.
+// far-field acceleration of slab row `row`; near_out (or null) receives the
+// near-field record (xyz * inv_eps3, w) at row `row`.
 struct KickArgs {
     const float *far;
+    float *near_out;
     int64_t far_base;
     float rescale, inv_eps3;
     double density_rad2;

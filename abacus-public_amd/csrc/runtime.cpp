@@ -375,6 +375,7 @@ size_t task_bytes(const nf_sic &s, int path, int32_t n_chunks) {
     if (s.mem == NF_MEM_HOST) {
         for (int c = 0; c < s.nfwidth; c++) b += align_up(12 * (size_t)s.Nslab[c], 256);
         b += align_up(16 * (size_t)s.Nslab[s.nfradius], 256);
+        if (s.kick && s.kick->NearAccOut) b += align_up(16 * (size_t)s.Nslab[s.nfradius], 256);
     }
     return b;
 }
@@ -895,7 +896,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     // slab cache), plus a device acceleration slab per SIC.
     std::vector<const float *> hslab;
     std::vector<uint64_t> hslab_n;
-    std::vector<size_t> acc_off(n, 0);
+    std::vector<size_t> acc_off(n, 0), near_off(n, 0);
     size_t slab_total = 0;
     if (s0.mem == NF_MEM_HOST) {
         for (int i = 0; i < n; i++) {
@@ -907,6 +908,10 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             }
             acc_off[i] = slab_total;
             slab_total = align_up(slab_total + 16 * (size_t)s.Nslab[s.nfradius], 256);
+            if (s.kick && s.kick->NearAccOut) {  // and one for the near-field record
+                near_off[i] = slab_total;
+                slab_total = align_up(slab_total + 16 * (size_t)s.Nslab[s.nfradius], 256);
+            }
         }
     }
     if (plan_total + work_total + slab_total > rt.max_buffer_bytes)
@@ -1021,6 +1026,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             ka.inv_eps3 = (float)s.kick->inv_eps3;
             ka.density_rad2 = s.kick->density_rad2;
             ka.far = s.mem == NF_MEM_HOST ? reinterpret_cast<const float *>(wb + WL[i].o_far) : s.kick->FarAccSlab;
+            if (s.kick->NearAccOut)
+                ka.near_out = s.mem == NF_MEM_HOST ? reinterpret_cast<float *>(db + near_off[i]) : s.kick->NearAccOut;
             ka.far_base = s.mem == NF_MEM_HOST ? (int64_t)r0 : 0;
         }
 
@@ -1230,18 +1237,26 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
             // The copy-back runs on the download stream, after the group's
             // kernels, while the next group's kernels use the compute stream:
             // straight into a pinned acceleration slab, else through staging.
-            float *dst = s.SinkAccSlab + 4 * r0;
             const size_t nb = 16 * (size_t)(r1 - r0);
-            const bool direct = host_is_pinned(dst);
-            if (!direct && (rc = w.pin_acc.ensure(nb))) return rc;
-            NF_HIP(hipEventRecord(w.ev[0], rt.cache.down));
-            NF_HIP(hipMemcpyAsync(direct ? (void *)dst : w.pin_acc.p, accd[i] + 4 * r0, nb, hipMemcpyDeviceToHost,
-                                  rt.cache.down));
-            NF_HIP(hipEventRecord(w.ev[2], rt.cache.down));
-            NF_HIP(hipEventSynchronize(w.ev[2]));
-            download_ms += event_ms(w.ev[0], w.ev[2]);
-            if (!direct) par_memcpy(dst, w.pin_acc.p, nb);
-            from_dev += nb;
+            auto copy_rows = [&](float *slab, const float *dev) -> int {
+                float *dst = slab + 4 * r0;
+                const bool direct = host_is_pinned(dst);
+                int rc2 = NF_OK;
+                if (!direct && (rc2 = w.pin_acc.ensure(nb))) return rc2;
+                NF_HIP(hipEventRecord(w.ev[0], rt.cache.down));
+                NF_HIP(hipMemcpyAsync(direct ? (void *)dst : w.pin_acc.p, dev + 4 * r0, nb, hipMemcpyDeviceToHost,
+                                      rt.cache.down));
+                NF_HIP(hipEventRecord(w.ev[2], rt.cache.down));
+                NF_HIP(hipEventSynchronize(w.ev[2]));
+                download_ms += event_ms(w.ev[0], w.ev[2]);
+                if (!direct) par_memcpy(dst, w.pin_acc.p, nb);
+                from_dev += nb;
+                return NF_OK;
+            };
+            if ((rc = copy_rows(s.SinkAccSlab, accd[i]))) return rc;
+            if (s.kick && s.kick->NearAccOut &&
+                (rc = copy_rows(s.kick->NearAccOut, reinterpret_cast<const float *>(db + near_off[i]))))
+                return rc;
         }
         copyback = std::chrono::duration<double>(clk::now() - tc).count();
     }

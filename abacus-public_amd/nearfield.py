@@ -67,7 +67,7 @@ class nf_sic(C.Structure):
 
 class nf_kick(C.Structure):
     _fields_ = [("FarAccSlab", _P), ("rescale", C.c_double), ("inv_eps3", C.c_double),
-                ("density_rad2", C.c_double)]
+                ("density_rad2", C.c_double), ("NearAccOut", _P)]
 
 
 NF_POS_XYZ, NF_POS_AOS = 0, 1

@@ -68,7 +68,8 @@ typedef enum nf_pos_layout { NF_POS_XYZ = 0, NF_POS_AOS = 1 } nf_pos_layout;
 
 /* Fusion of the consumer RescaleAndCoAddAcceleration (kick.cpp:74-109) into
  * the co-add; no counterpart in the reference's GPU module.  With it the SIC's
- * sink rows of SinkAccSlab receive the final acceleration,
+ * sink rows of SinkAccSlab receive the final acceleration (and, if asked, the
+ * near-field record StoreForces == 2 writes to disk goes to NearAccOut),
  *   acc.xyz = (near.xyz * inv_eps3 + FarAcc.xyz) * rescale,
  *   acc.w   = near.w - density_rad2
  * (FLOAT3p1 arithmetic: w takes no part in the scaling, float3p1.cpp:41-56),
@@ -81,6 +82,10 @@ typedef struct nf_kick {
     double inv_eps3;         /* 1 / SofteningLengthInternal^3 for the single spline, else 1;
                                 rounded to float */
     double density_rad2;     /* WriteState.DensityKernelRad2 (double, as the reference) */
+    float *NearAccOut;       /* optional (NULL: none), float4 per particle of slab SlabId, in the
+                                SIC's nf_mem: the StoreForces == 2 NearAcc record of the SIC's
+                                rows, near.xyz * inv_eps3 and near.w (pipeline.cpp:374-399,
+                                FLOAT3p1 *= FLOAT leaves w alone) */
 } nf_kick;
 
 /* CellPencilPlan (SetInteractionCollection.hh:76-87): start index of a cell

@@ -227,3 +227,15 @@ def rescale_coadd(nacc, facc, rescale, inv_eps3, density_rad2):
     out[:, :3] = ((n[:, :3] * ie) + f) * r
     out[:, 3] = (n[:, 3].astype(np.float64) - float(density_rad2)).astype(np.float32)
     return out
+
+
+def near_acc_record(nacc, inv_eps3):
+    """The NearAcc record StoreForces == 2 writes for a slab (pipeline.cpp:374-399):
+    the AccSlab with the single-spline prefactor applied in place, nearacc[i] *=
+    inv_eps3 with a FLOAT inv_eps3 -- FLOAT3p1 *= FLOAT (float3p1.cpp:50-51)
+    scales x, y, z and leaves w.  Other softenings write the AccSlab as it is,
+    which is inv_eps3 = 1."""
+    n = np.asarray(nacc, np.float32).reshape(-1, 4)
+    out = n.copy()
+    out[:, :3] = n[:, :3] * np.float32(inv_eps3)
+    return out

@@ -52,13 +52,14 @@ def oracle_slab(box, x, softening, eps, b2, with_soft=False):
 
 
 def run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_XYZ, kick_args=None, planner="host",
-             with_stats=False):
+             with_stats=False, near_out=False):
     """Whole-slab SICs for ``slabs`` in host or device mode; positions in the
     given layout, an optional fused kick per slab (far-field slab, constants).
     planner "host": nf_plan_sic + nf_submit; "device": nf_submit_slab.
-    Returns {slab: (n, 4) float32}; with_stats=True also the tasks' stats."""
+    Returns {slab: (n, 4) float32}; with_stats=True also the tasks' stats;
+    near_out=True (with a kick) also the NearAccOut records {slab: (n, 4)}."""
     cpd = box.cpd
-    keep, out, tasks, stats = [], {}, [], []
+    keep, out, tasks, stats, near = [], {}, [], [], {}
     pos = {}
     for x in range(cpd):
         s = box.slab(x)
@@ -83,6 +84,14 @@ def run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_XYZ, kick_args=None, pl
                 keep.append(far)
                 fp = far.ctypes.data
             kick = nfm.nf_kick(FarAccSlab=fp, rescale=rescale, inv_eps3=inv_eps3, density_rad2=dens)
+            if near_out:
+                if mem == nfm.NF_MEM_DEVICE:
+                    near[x] = nfm.DeviceArray(nbytes=16 * n)
+                    keep.append(near[x])
+                    kick.NearAccOut = near[x].ptr
+                else:
+                    near[x] = np.full((n, 4), np.nan, np.float32)
+                    kick.NearAccOut = near[x].ctypes.data
         if mem == nfm.NF_MEM_DEVICE:
             acc = nfm.DeviceArray(nbytes=16 * n)
             keep.append(acc)
@@ -104,4 +113,7 @@ def run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_XYZ, kick_args=None, pl
         t.release()
     res = {x: (a.to_host(np.float32, (box.slab(x).n, 4)) if mem == nfm.NF_MEM_DEVICE else a)
            for x, a in out.items()}
+    if near_out:
+        return res, {x: (a.to_host(np.float32, (box.slab(x).n, 4)) if mem == nfm.NF_MEM_DEVICE else a)
+                     for x, a in near.items()}
     return (res, stats) if with_stats else res

@@ -43,6 +43,7 @@ def test_struct_layout_matches_c(tmp_path):
     for f in slab_fields:
         src.append(f'printf("%zu\\n", offsetof(nf_slab_task, {f}));')
     src.append('printf("%zu\\n", sizeof(nf_slab_task));')
+    src.append('printf("%zu\\n", offsetof(nf_kick, NearAccOut));')
     src.append('printf("%zu %zu %zu\\n", offsetof(nf_kick, density_rad2), sizeof(nf_kick), sizeof(nf_setup_params));'
                ' return 0; }')
     c = tmp_path / "probe.c"
@@ -60,8 +61,8 @@ def test_struct_layout_matches_c(tmp_path):
     k = 5 + len(fields)
     offs = list(map(int, out[k:k + len(slab_fields) + 1]))
     assert offs == [getattr(nfm.nf_slab_task, f).offset for f in slab_fields] + [C.sizeof(nfm.nf_slab_task)]
-    assert list(map(int, out[-3:])) == [nfm.nf_kick.density_rad2.offset, C.sizeof(nfm.nf_kick),
-                                        C.sizeof(nfm.nf_setup_params)]
+    assert list(map(int, out[-4:])) == [nfm.nf_kick.NearAccOut.offset, nfm.nf_kick.density_rad2.offset,
+                                        C.sizeof(nfm.nf_kick), C.sizeof(nfm.nf_setup_params)]
 
 
 def test_plan_errors_are_codes():

@@ -362,7 +362,8 @@ def test_aos_positions_match_posxyz(box9, path, mem):
 def test_fused_kick_matches_consumer(box9, path, mem):
     """With the kick fused into the co-add, the acceleration slab holds
     bitwise what RescaleAndCoAddAcceleration (the oracle's restatement) makes
-    of the unfused near field and the far field."""
+    of the unfused near field and the far field, and NearAccOut, if given,
+    bitwise the NearAcc record StoreForces == 2 writes."""
     eps, b2, e = params(box9)
     slabs = [1, 4, 7]
     rng = np.random.default_rng(5)
@@ -375,7 +376,11 @@ def test_fused_kick_matches_consumer(box9, path, mem):
                              max_buffer_bytes=2 << 30, sinks_per_lane=2) as drv:
         near = run_sics(drv, box9, slabs, mem)
         fused = run_sics(drv, box9, slabs, mem, kick_args=kick_args)
+        fused2, record = run_sics(drv, box9, slabs, mem, kick_args=kick_args, near_out=True)
     for x in slabs:
         expect = po.rescale_coadd(near[x], kick_args[x][0], rescale, inv_eps3, dens)
         assert np.array_equal(fused[x], expect), x
+        # with NearAccOut the co-add also writes the StoreForces == 2 record
+        assert np.array_equal(fused2[x], expect), x
+        assert np.array_equal(record[x], po.near_acc_record(near[x], inv_eps3)), x
         assert np.abs(fused[x][:, 3]).max() < 10 * dens      # w minus its self term is small

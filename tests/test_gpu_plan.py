@@ -93,12 +93,14 @@ def test_submit_slab_matches_submit(path, mem):
                              max_buffer_bytes=2 << 30, sinks_per_lane=2) as drv:
         host = run_sics(drv, box, slabs, mem)
         dev, st = run_sics(drv, box, slabs, mem, planner="device", with_stats=True)
-        hk = run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_AOS, kick_args=kick_args)
-        dk = run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_AOS, kick_args=kick_args, planner="device")
+        hk, hn = run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_AOS, kick_args=kick_args, near_out=True)
+        dk, dn = run_sics(drv, box, slabs, mem, pos_layout=nfm.NF_POS_AOS, kick_args=kick_args, planner="device",
+                          near_out=True)
     for i, x in enumerate(slabs):
         assert np.isfinite(dev[x]).all()
         assert np.array_equal(host[x], dev[x]), x
         assert np.array_equal(hk[x], dk[x]), x
+        assert np.isfinite(dn[x]).all() and np.array_equal(hn[x], dn[x]), x
         sic = nfm.SetInteractionCollection(box, x, 0, box.cpd, R, eps, b2, "single_spline")
         assert st[i]["device_planned"] == 1 and st[i]["plan_s"] > 0
         assert st[i]["direct_total"] == sic.desc.DirectTotal

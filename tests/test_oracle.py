@@ -268,3 +268,21 @@ def test_rescale_coadd_known_answers():
     o = po.rescale_coadd(np.array([[1.0, 0, 0, 1.0]], np.float32), np.zeros((1, 3), np.float32),
                          1.0 / 3.0, 1.0, 1e-9)
     assert o[0, 0] == np.float32(1.0 / 3.0) and o[0, 3] == np.float32(1.0 - 1e-9)
+
+
+def test_near_acc_record_known_answers():
+    """The StoreForces == 2 NearAcc record: xyz scaled by the float inv_eps3,
+    w left alone (FLOAT3p1 *= FLOAT); inv_eps3 = 1 (softenings without the
+    prefactor) writes the AccSlab unchanged."""
+    nacc = np.array([[1.0, -2.0, 0.5, 3.0], [0.0, 7.0, -1.0, 0.25]], np.float32)
+    np.testing.assert_array_equal(po.near_acc_record(nacc, 4.0),
+                                  np.array([[4.0, -8.0, 2.0, 3.0], [0.0, 28.0, -4.0, 0.25]], np.float32))
+    np.testing.assert_array_equal(po.near_acc_record(nacc, 1.0), nacc)
+    # inv_eps3 is a FLOAT: 1/3 rounds to float32 before the product
+    o = po.near_acc_record(np.array([[3.0, 0, 0, 5.0]], np.float32), 1.0 / 3.0)
+    assert o[0, 0] == np.float32(3.0) * np.float32(1.0 / 3.0) and o[0, 3] == np.float32(5.0)
+    # the fused kick's xyz is the record's xyz carried on: (record + far) * rescale
+    far = np.array([[0.5, 0.5, -0.5], [1.0, 2.0, 4.0]], np.float32)
+    rec = po.near_acc_record(nacc, 4.0)
+    np.testing.assert_array_equal(po.rescale_coadd(nacc, far, -2.0, 4.0, 0.0)[:, :3],
+                                  ((rec[:, :3] + far) * np.float32(-2.0)).astype(np.float32))
