@@ -68,39 +68,59 @@ __device__ __forceinline__ float3 slab_xyz(const float *P, uint64_t n, uint64_t 
     return make_float3(P[idx], P[n + idx], P[2 * n + idx]);
 }
 
+// Padded blocks per gather wave.  A block's chain of dependent loads (its
+// parent pencil, the pencil's plan, then the positions) is short on work;
+// a wave that walks four blocks with their loads interleaved keeps four
+// chains in flight.
+constexpr int kGatherBlocks = 4;
+
 template <bool F64>
 __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *__restrict__ slab, uint64_t n,
                                                       int aos, float4 *__restrict__ out, float cellsize,
                                                       double invcpd) {
-    const int b = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
-    if (b >= pl.nsink_blocks) return;
+    const int b0 = (blockIdx.x * 4 + (int)(threadIdx.x / kWave)) * kGatherBlocks;
+    if (b0 >= pl.nsink_blocks) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int s = pl.parent[b];
     const int W = pl.W;
-    const nf_sink_plan p = pl.sink_plan[s];
-    int pre[NF_MAX_WIDTH + 1];
-    pre[0] = 0;
-    for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
-    const int count = pre[W];
-    const int i = b * kWave - pl.sink_set_start[s] + lane;  // index within the pencil
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (i < count) {
-        const int c = cell_of(i, pre, W);
-        const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
-        const float3 q = slab_xyz(slab, n, idx, aos);
-        v.x = q.x;
-        v.y = q.y;
-        float z = q.z;
-        const int dc = c - pl.R;
-        if (F64) {
-            z = (float)((double)z + (double)dc * invcpd);
-        } else {  // PencilPlan.cc:43-50: dz = (c-R)*cellsize in FLOAT, added if nonzero
-            const float dz = (float)dc * cellsize;
-            if (dz != 0.f) z = z + dz;
-        }
-        v.z = z;
+    int par[kGatherBlocks];
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++) par[u] = pl.parent[min(b0 + u, pl.nsink_blocks - 1)];
+    uint64_t idx[kGatherBlocks];
+    int dcell[kGatherBlocks];
+    bool live[kGatherBlocks];
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++) {
+        const int s = par[u];
+        const nf_sink_plan p = pl.sink_plan[s];
+        int pre[NF_MAX_WIDTH + 1];
+        pre[0] = 0;
+        for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
+        const int i = (b0 + u) * kWave - pl.sink_set_start[s] + lane;  // index within the pencil
+        live[u] = b0 + u < pl.nsink_blocks && i < pre[W];
+        const int c = live[u] ? cell_of(i, pre, W) : 0;
+        idx[u] = live[u] ? (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]) : 0;
+        dcell[u] = c - pl.R;
     }
-    out[(int64_t)b * kWave + lane] = v;
+    float3 q[kGatherBlocks];
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++)
+        q[u] = live[u] ? slab_xyz(slab, n, idx[u], aos) : make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++) {
+        if (b0 + u >= pl.nsink_blocks) break;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live[u]) {
+            float z = q[u].z;
+            if (F64) {
+                z = (float)((double)z + (double)dcell[u] * invcpd);
+            } else {  // PencilPlan.cc:43-50: dz = (c-R)*cellsize in FLOAT, added if nonzero
+                const float dz = (float)dcell[u] * cellsize;
+                if (dz != 0.f) z = z + dz;
+            }
+            v = make_float4(q[u].x, q[u].y, z, 0.f);
+        }
+        out[(int64_t)(b0 + u) * kWave + lane] = v;
+    }
 }
 
 __device__ __forceinline__ const float *slab_ptr(const DevSlabs &sl, int c) {
@@ -119,47 +139,68 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// One wave per padded 64-source block; it writes the block and its bounding
-// box (padding excluded): box[2b] = min, box[2b+1] = max.
+// One wave per kGatherBlocks padded 64-source blocks; it writes each block
+// and its bounding box (padding excluded): box[2b] = min, box[2b+1] = max.
 template <bool F64>
 __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
                                                         float4 *__restrict__ box, float cellsize,
                                                         double invcpd) {
-    const int b = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
-    if (b >= pl.nsrc_blocks) return;
+    const int b0 = (blockIdx.x * 4 + (int)(threadIdx.x / kWave)) * kGatherBlocks;
+    if (b0 >= pl.nsrc_blocks) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int s = pl.src_parent[b];
-    if (s < 0) return;  // a block no pencil covers (never read)
     const int W = pl.W;
-    const nf_source_plan p = pl.src_plan[s];
-    int pre[NF_MAX_WIDTH + 1];
-    pre[0] = 0;
-    for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
-    const int count = pre[W];
-    const int i = b * kWave - pl.src_set_start[s] + lane;
-    float4 v = make_float4(kFar, 0.f, 0.f, 0.f);
-    const bool real = i < count;
-    if (real) {
-        const int c = cell_of(i, pre, W);
-        const uint64_t idx = (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]);
-        const float3 q = slab_xyz(slab_ptr(sl, c), slab_n(sl, c), idx, sl.aos);
-        float x = q.x;
-        const int dc = c - pl.R;
-        if (F64) {
-            x = (float)((double)x + (double)dc * invcpd);
-        } else {  // PencilPlan.cc:226-238
-            const float dx = (float)dc * cellsize;
-            if (dx != 0.f) x = x + dx;
-        }
-        v = make_float4(x, q.y, q.z, 0.f);
+    int par[kGatherBlocks];
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++) par[u] = pl.src_parent[min(b0 + u, pl.nsrc_blocks - 1)];
+    uint64_t idx[kGatherBlocks];
+    int cc[kGatherBlocks];
+    bool live[kGatherBlocks];
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++) {
+        const int s = max(par[u], 0);  // a block no pencil covers (s < 0) is never read: left unwritten
+        const nf_source_plan p = pl.src_plan[s];
+        int pre[NF_MAX_WIDTH + 1];
+        pre[0] = 0;
+        for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
+        const int i = (b0 + u) * kWave - pl.src_set_start[s] + lane;
+        live[u] = b0 + u < pl.nsrc_blocks && par[u] >= 0 && i < pre[W];
+        const int c = live[u] ? cell_of(i, pre, W) : 0;
+        idx[u] = live[u] ? (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]) : 0;
+        cc[u] = c;
     }
-    out[(int64_t)b * kWave + lane] = v;
+    float3 q[kGatherBlocks];
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++)
+        q[u] = live[u] ? slab_xyz(slab_ptr(sl, cc[u]), slab_n(sl, cc[u]), idx[u], sl.aos)
+                       : make_float3(0.f, 0.f, 0.f);
     const float inf = __builtin_huge_valf();
-    const float mnx = wave_min(real ? v.x : inf), mny = wave_min(real ? v.y : inf), mnz = wave_min(real ? v.z : inf);
-    const float mxx = wave_max(real ? v.x : -inf), mxy = wave_max(real ? v.y : -inf), mxz = wave_max(real ? v.z : -inf);
-    if (lane == 0) {
-        box[2 * (int64_t)b] = make_float4(mnx, mny, mnz, 0.f);
-        box[2 * (int64_t)b + 1] = make_float4(mxx, mxy, mxz, 0.f);
+#pragma unroll
+    for (int u = 0; u < kGatherBlocks; u++) {
+        const int b = b0 + u;
+        if (b >= pl.nsrc_blocks) break;
+        if (par[u] < 0) continue;
+        float4 v = make_float4(kFar, 0.f, 0.f, 0.f);
+        const bool real = live[u];
+        if (real) {
+            float x = q[u].x;
+            const int dc = cc[u] - pl.R;
+            if (F64) {
+                x = (float)((double)x + (double)dc * invcpd);
+            } else {  // PencilPlan.cc:226-238
+                const float dx = (float)dc * cellsize;
+                if (dx != 0.f) x = x + dx;
+            }
+            v = make_float4(x, q[u].y, q[u].z, 0.f);
+        }
+        out[(int64_t)b * kWave + lane] = v;
+        const float mnx = wave_min(real ? v.x : inf), mny = wave_min(real ? v.y : inf),
+                    mnz = wave_min(real ? v.z : inf);
+        const float mxx = wave_max(real ? v.x : -inf), mxy = wave_max(real ? v.y : -inf),
+                    mxz = wave_max(real ? v.z : -inf);
+        if (lane == 0) {
+            box[2 * (int64_t)b] = make_float4(mnx, mny, mnz, 0.f);
+            box[2 * (int64_t)b + 1] = make_float4(mxx, mxy, mxz, 0.f);
+        }
     }
 }
 
@@ -422,6 +463,41 @@ struct BatchCtx {
     float dens2;  // squared distance above which the FOF density term is zero (with margin)
 };
 
+// A sink pencil's cells, for reading its sinks straight from the slab.
+struct SinkCells {
+    uint32_t start[NF_MAX_WIDTH];
+    int pre[NF_MAX_WIDTH + 1];  // sinks before each cell
+};
+__device__ __forceinline__ SinkCells sink_cells(const SicView &v, int s) {
+    const nf_sink_plan p = cptr<nf_sink_plan>(v.sink_plan)[s];
+    SinkCells sc;
+    sc.pre[0] = 0;
+#pragma unroll
+    for (int c = 0; c < NF_MAX_WIDTH; c++) {
+        sc.start[c] = p.cell[c].start;
+        sc.pre[c + 1] = sc.pre[c] + (c < v.W ? p.cell[c].N : 0);
+    }
+    return sc;
+}
+// Sink i of the pencil exactly as k_gather_sinks<true> puts it in a sink
+// stream: its slab position with z moved into the pencil's central cell by
+// the FP64 offset (c - R)/cpd.
+__device__ __forceinline__ float3 pencil_sink(const SicView &v, const SinkCells &sc, int i, double invcpd) {
+    // The cell holding sink i, by selects over the (wave-uniform) cell table:
+    // no indexed private array.
+    int c = 0, pre = 0;
+    uint32_t start = sc.start[0];
+#pragma unroll
+    for (int cc = 1; cc < NF_MAX_WIDTH; cc++) {
+        const bool in = cc < v.W && i >= sc.pre[cc];
+        c = in ? cc : c;
+        pre = in ? sc.pre[cc] : pre;
+        start = in ? sc.start[cc] : start;
+    }
+    const float3 q = slab_xyz(v.sink_slab, v.sink_n, (uint64_t)start + (uint64_t)(i - pre), v.aos);
+    return make_float3(q.x, q.y, (float)((double)q.z + (double)(c - v.R) * invcpd));
+}
+
 // The mode a source tile needs, from the squared gap between its bounding box
 // and the sinks' bounding box (wave-uniform).
 template <int SOFT>
@@ -466,7 +542,6 @@ __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, c
 // uniform and come through the scalar cache.
 template <int SOFT, int S>
 __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane) {
-    const NF_CONST float4 *sinks = cptr<float4>(v.sinks4);
     const NF_CONST float4 *src = cptr<float4>(v.sources4);
     const NF_CONST float4 *box = cptr<float4>(v.box4);
     const NF_CONST int32_t *sink_set_start = cptr<int32_t>(v.sink_set_start);
@@ -477,13 +552,14 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
     const int s = w.sinkset;
     const int base = sink_set_start[s] + w.first;
     const int nsink = w.nsink;  // 64*S, or fewer for a masked tail tile
+    const SinkCells sc = sink_cells(v, s);
     float kx[S], ky[S], kz[S];
     float blx = __builtin_huge_valf(), bly = blx, blz = blx, bhx = -blx, bhy = -blx, bhz = -blx;
 #pragma unroll
     for (int u = 0; u < S; u++) {
         // Lanes past nsink repeat sink 0 (inside the box, results not stored).
         const int ki = lane + kWave * u < nsink ? lane + kWave * u : 0;
-        const float4 k = sinks[base + ki];
+        const float3 k = pencil_sink(v, sc, w.first + ki, cx.invcpd);
         kx[u] = k.x;
         ky[u] = k.y;
         kz[u] = k.z;
@@ -579,7 +655,6 @@ __device__ __forceinline__ float group_sum(float v) {
 template <int SOFT, int SU>
 __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane) {
     constexpr int TC = 16;  // 64-source blocks per TwoSum tile
-    const NF_CONST float4 *sinks = cptr<float4>(v.sinks4);
     const NF_CONST float4 *box = cptr<float4>(v.box4);
     const NF_CONST int32_t *sink_set_start = cptr<int32_t>(v.sink_set_start);
     const NF_CONST int32_t *src_set_start = cptr<int32_t>(v.src_set_start);
@@ -592,6 +667,7 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
     const int r = w.nsink;
     const int W = v.W;
     const int grp = lane / kGroupLanes, gl = lane % kGroupLanes;
+    const SinkCells sc = sink_cells(v, s);
     for (int g0 = 0; g0 < r; g0 += kGroups * SU) {
         // This group's sinks: g0 + grp*SU + u (repeats of the last sink past
         // the tail are computed and not stored).
@@ -600,7 +676,7 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
 #pragma unroll
         for (int u = 0; u < SU; u++) {
             const int gi = min(g0 + grp * SU + u, r - 1);
-            const float4 k = sinks[base + gi];
+            const float3 k = pencil_sink(v, sc, w.first + gi, cx.invcpd);
             kx[u] = k.x;
             ky[u] = k.y;
             kz[u] = k.z;
@@ -765,8 +841,11 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void
     const float cellsize = (float)(1.0 / (double)pl.cpd);
     const double invcpd = 1.0 / (double)pl.cpd;
     float4 *box = static_cast<float4 *>(box_after(sources4, pl.nsrc_blocks));
-    if (pl.nsink_blocks > 0) {
-        const dim3 grid((pl.nsink_blocks + 3) / 4);
+    auto grid_of = [](int blocks) {  // 4 waves per workgroup, kGatherBlocks blocks per wave
+        return dim3((unsigned)((blocks + 4 * kGatherBlocks - 1) / (4 * kGatherBlocks)));
+    };
+    if (sinks4 && pl.nsink_blocks > 0) {
+        const dim3 grid = grid_of(pl.nsink_blocks);
         if (fp64_offsets)
             hipLaunchKernelGGL(k_gather_sinks<true>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R], sl.aos,
                                static_cast<float4 *>(sinks4), cellsize, invcpd);
@@ -775,7 +854,7 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void
                                static_cast<float4 *>(sinks4), cellsize, invcpd);
     }
     if (pl.nsrc_blocks > 0) {
-        const dim3 grid((pl.nsrc_blocks + 3) / 4);
+        const dim3 grid = grid_of(pl.nsrc_blocks);
         if (fp64_offsets)
             hipLaunchKernelGGL(k_gather_sources<true>, grid, dim3(256), 0, st, pl, sl, static_cast<float4 *>(sources4),
                                box, cellsize, invcpd);

@@ -33,12 +33,15 @@ struct SicView {
     const int32_t *src_set_start;
     const int32_t *src_set_count;
     const int32_t *ilist;
-    const void *sinks4;    // padded sink stream, float4
-    const void *sources4;  // padded source stream, float4, followed by 2 float4 boxes per block
+    const nf_sink_plan *sink_plan;  // sinks are read from the slab through the pencil plans
+    const float *sink_slab;         // position slab of SlabId (layout: aos)
+    uint64_t sink_n;                // its particle count
+    const void *sources4;           // padded source stream, float4, followed by 2 float4 boxes per block
     const void *box4;
-    double *partial;       // 4 doubles per padded sink slot
+    double *partial;                // 4 doubles per padded sink slot
     int32_t W, R;
-    int32_t pad[2];
+    int32_t aos;                    // 1: posstruct (x,y,z) per particle; 0: PosXYZ X|Y|Z
+    int32_t pad;
 };
 
 // Device views of the plan arrays of one SIC.
@@ -104,6 +107,8 @@ struct TileShape {
 
 // Kernel launchers (kernels.hip).  All enqueue on `stream` and return a HIP
 // error code (0 on success).
+// Source pencils always; sink pencils only if sinks4 is non-null (the plain
+// path; the batched kernel reads its sinks from the slab itself).
 int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void *sinks4,
                   void *sources4, void *stream);
 int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const void *sinks4,

@@ -340,8 +340,8 @@ WorkLayout work_layout(const nf_sic &s, int path, size_t base) {
     WorkLayout L{};
     const size_t psink = (size_t)s.NSinkBlocks * NF_BLOCK_SIZE;
     const size_t psrc = (size_t)s.NSourceBlocks * NF_BLOCK_SIZE;
-    L.o_sinks = base;
-    L.o_src = align_up(L.o_sinks + 16 * psink, 256);
+    L.o_sinks = base;  // the plain path's sink stream (the batched kernel reads the slab)
+    L.o_src = align_up(L.o_sinks + (path == NF_PATH_BATCHED ? 0 : 16 * psink), 256);
     L.o_part = align_up(L.o_src + 16 * psrc + 32 * (size_t)s.NSourceBlocks, 256);
     size_t end = align_up(L.o_part + psink * (path == NF_PATH_BATCHED ? 32 : 16), 256);
     L.o_far = end;  // host mode with a fused kick: the sink rows' far-field accelerations
@@ -1037,7 +1037,10 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         v.src_set_start = pl.src_set_start;
         v.src_set_count = pl.src_set_count;
         v.ilist = pl.ilist;
-        v.sinks4 = wb + WL[i].o_sinks;
+        v.sink_plan = pl.sink_plan;
+        v.sink_slab = sl.pos[s.nfradius];
+        v.sink_n = s.Nslab[s.nfradius];
+        v.aos = sl.aos;
         v.sources4 = wb + WL[i].o_src;
         v.box4 = box_after(wb + WL[i].o_src, s.NSourceBlocks);
         v.partial = reinterpret_cast<double *>(wb + WL[i].o_part);
@@ -1135,7 +1138,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     }
     int e = 0;
     for (int i = 0; i < n && !e; i++)
-        e = launch_gather(plans[i], slabs[i], path == NF_PATH_BATCHED, wb + WL[i].o_sinks, wb + WL[i].o_src, stream);
+        e = launch_gather(plans[i], slabs[i], path == NF_PATH_BATCHED,
+                          path == NF_PATH_BATCHED ? nullptr : wb + WL[i].o_sinks, wb + WL[i].o_src, stream);
     if (e) return fail(NF_ERR_HIP, std::string("gather launch: ") + hipGetErrorString((hipError_t)e));
     if (!use_graph) NF_HIP(hipEventRecord(w.ev[2], stream));
     std::vector<hipEvent_t> tev;
