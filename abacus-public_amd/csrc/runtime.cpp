@@ -407,34 +407,52 @@ size_t queued_bytes(const Task &t, int path) {
 }
 uint64_t queued_interactions(const Task &t) { return t.dp.on ? dp_interactions_estimate(t.sic) : t.sic.DirectTotal; }
 
-// The batched work list of one SIC: per sink pencil, full register tiles of
-// 64*S sinks, then the tail; cost = sinks x sources of the item.
-void build_items(const nf_sic &s, int S, int sic, std::vector<WorkItem> &items, std::vector<uint64_t> &cost) {
+// The 1/8-octave bucket of an item's cost (sinks x sources), longest first:
+// bucket 0 holds the most expensive items, kItemBuckets - 1 the cheapest.
+constexpr int kItemBuckets = 1024;
+uint16_t cost_bucket(uint64_t cost) {
+    const uint64_t b = cost ? (uint64_t)std::floor(8.0 * std::log2((double)cost)) : 0;
+    return (uint16_t)(kItemBuckets - 1 - std::min<uint64_t>(b, kItemBuckets - 1));
+}
+
+// The batched work list of one SIC, appended in sink-pencil order z then y
+// (pencils adjacent in y share 2R of their 2R+1 source pencils), with each
+// item's cost bucket: per sink pencil, full register tiles of 64*S sinks,
+// then the tail.
+void build_items(const nf_sic &s, int S, int sic, std::vector<WorkItem> &items, std::vector<uint16_t> &bucket) {
     const int tile = NF_BLOCK_SIZE * S;
     const int W = s.nfwidth;
-    for (int k = 0; k < s.NSinkSets; k++) {
-        const int n = s.SinkSetCount[k];
-        if (n == 0) continue;
-        uint64_t nsrc = 0;
-        for (int y = 0; y < W; y++) nsrc += (uint64_t)s.SourceSetCount[s.SinkSourceInteractionList[W * k + y]];
-        int f = 0;
-        for (; f + tile <= n; f += tile) {
-            items.push_back(WorkItem{k, f, tile, 0 + 4 * sic});
-            cost.push_back((uint64_t)tile * nsrc);
+    for (int z = 0; z < s.Nk; z++)
+        for (int j = 0; j < s.j_width; j++) {
+            const int k = j * s.Nk + z;
+            const int n = s.SinkSetCount[k];
+            if (n == 0) continue;
+            uint64_t nsrc = 0;
+            for (int y = 0; y < W; y++) nsrc += (uint64_t)s.SourceSetCount[s.SinkSourceInteractionList[W * k + y]];
+            int f = 0;
+            if (n >= tile) {
+                const uint16_t b = cost_bucket((uint64_t)tile * nsrc);
+                for (; f + tile <= n; f += tile) {
+                    items.push_back(WorkItem{k, f, tile, 0 + 4 * sic});
+                    bucket.push_back(b);
+                }
+            }
+            // The tail: one-wave tiles while they keep at least kTailLanes
+            // lanes busy, then the rest with sources on the lanes.
+            if (n - f >= kTailLanes) {
+                const uint16_t b = cost_bucket((uint64_t)NF_BLOCK_SIZE * nsrc);
+                while (n - f >= kTailLanes) {
+                    const int m = std::min(NF_BLOCK_SIZE, n - f);
+                    items.push_back(WorkItem{k, f, m, 2 + 4 * sic});
+                    bucket.push_back(b);
+                    f += m;
+                }
+            }
+            if (f < n) {
+                items.push_back(WorkItem{k, f, n - f, 1 + 4 * sic});
+                bucket.push_back(cost_bucket((uint64_t)(n - f) * nsrc));
+            }
         }
-        // The tail: one-wave tiles while they keep at least kTailLanes lanes
-        // busy, then the rest with sources on the lanes.
-        while (n - f >= kTailLanes) {
-            const int m = std::min(NF_BLOCK_SIZE, n - f);
-            items.push_back(WorkItem{k, f, m, 2 + 4 * sic});
-            cost.push_back((uint64_t)NF_BLOCK_SIZE * nsrc);
-            f += m;
-        }
-        if (f < n) {
-            items.push_back(WorkItem{k, f, n - f, 1 + 4 * sic});
-            cost.push_back((uint64_t)(n - f) * nsrc);
-        }
-    }
 }
 
 int validate(const nf_sic &s) {
@@ -819,25 +837,16 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     // Work list over the whole group, longest items first: one list per
     // distinct sinks-per-lane value to be launched (several while tuning).
     // Items within 1/8 octave of cost count as equal and are kept in sink
-    // pencil order (SIC, z, y): pencils adjacent in y share 2R of their 2R+1
-    // source pencils.
+    // pencil order (SIC, z, y): a stable counting sort by bucket.
     auto make_items = [&](int S) {
         std::vector<WorkItem> raw, out;
-        std::vector<uint64_t> cost;
-        for (int i = 0; i < n; i++) build_items(g[i]->sic, S, i, raw, cost);
-        std::vector<uint64_t> key(raw.size());
-        for (size_t i = 0; i < raw.size(); i++) {
-            const nf_sic &s = g[raw[i].kind_sic >> 2]->sic;
-            const uint32_t jj = (uint32_t)raw[i].sinkset / (uint32_t)s.Nk, kz = (uint32_t)raw[i].sinkset % (uint32_t)s.Nk;
-            const uint64_t bucket = cost[i] ? (uint64_t)std::floor(8.0 * std::log2((double)cost[i])) : 0;
-            key[i] = ((1023 - std::min<uint64_t>(bucket, 1023)) << 52) | ((uint64_t)(raw[i].kind_sic >> 2) << 40) |
-                     ((uint64_t)kz << 20) | jj;
-        }
-        std::vector<size_t> order(raw.size());
-        for (size_t i = 0; i < order.size(); i++) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
+        std::vector<uint16_t> bucket;
+        for (int i = 0; i < n; i++) build_items(g[i]->sic, S, i, raw, bucket);
+        std::vector<size_t> start(kItemBuckets + 1, 0);
+        for (uint16_t b : bucket) start[b + 1]++;
+        for (int b = 0; b < kItemBuckets; b++) start[b + 1] += start[b];
         out.resize(raw.size());
-        for (size_t i = 0; i < order.size(); i++) out[i] = raw[order[i]];
+        for (size_t i = 0; i < raw.size(); i++) out[start[bucket[i]]++] = raw[i];
         return out;
     };
     std::vector<int> Svals;
