@@ -246,13 +246,14 @@ def kernel_name(path, sinks_per_lane, waves_per_eu=0, softening=1):
 
 
 def algorithmic_bytes(r, kd):
-    """Bytes a batched launch must move at least: every padded sink and source
-    (float4) read once, two float4 bounding boxes per 64-source block, and a
+    """Bytes a batched launch must move at least: every padded sink slot's
+    position read once from the slab (3 floats), every padded source (float4)
+    and the two float4 bounding boxes per 64-source block read once, and a
     float64x4 partial written per padded sink; for a launch's mean number of
     SICs of mean size."""
     sics_per_launch = kd["interactions_per_launch"] / (r["direct_total"] / r["n_sics"])
     sz = r["sic_sizes"]
-    per_sic = 16 * sz["padded_sinks"] + 16 * sz["padded_sources"] * (1 + 2 / 64) + 32 * sz["padded_sinks"]
+    per_sic = 12 * sz["padded_sinks"] + 16 * sz["padded_sources"] * (1 + 2 / 64) + 32 * sz["padded_sinks"]
     return per_sic * sics_per_launch
 
 
