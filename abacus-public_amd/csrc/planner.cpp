@@ -9,10 +9,14 @@
 //   ComputeSICSize                          PencilOnPencil.cc:130-150
 //   SinkPencilPlan::load                    PencilPlan.cc:95-138
 //   SourcePencilPlan::load                  PencilPlan.cc:260-279
+//   ExecuteSlabGPU's y-split                directdriver.cpp:248-303
+//   NumParticlesInSkewer, ComputeSinkBlocks,
+//   ComputeSourceBlocks                     PencilOnPencil.cc:90-126
 // The output arrays have exactly the reference's contents; they are carved
 // page-aligned from a caller-owned arena as posix_memalign_wrap does
 // (PencilOnPencil.cc:156-164).
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -255,5 +259,63 @@ extern "C" int nf_plan_sic(int32_t cpd, int32_t R, int32_t slab, int32_t j_low, 
     out->SourceTotal = SourceTotal;
     out->PaddedSinkTotal = NSinkBlocks * NF_BLOCK_SIZE;
     out->PaddedSourceTotal = NSourceBlocks * NF_BLOCK_SIZE;
+    return NF_OK;
+}
+
+// NearFieldDriver::ExecuteSlabGPU's y-split (directdriver.cpp:248-303) with
+// the skewer estimates NumParticlesInSkewer / ComputeSinkBlocks /
+// ComputeSourceBlocks (PencilOnPencil.cc:90-126).  Integer arithmetic as the
+// reference's: int particle counts, int block counts, useMaxSink scaled by a
+// double factor and truncated.
+extern "C" int nf_split_slab(int32_t cpd, int32_t R, const nf_cellinfo *const *cellinfo, const uint64_t *nslab,
+                             int32_t nbuffers, int32_t max_sink_blocks, int32_t max_source_blocks, int32_t *split,
+                             int32_t *nsplit) {
+    if (!cellinfo || !nslab || !split || !nsplit) return nf::fail(NF_ERR_INVALID, "nf_split_slab: null argument");
+    if (cpd <= 0 || (cpd % 2) == 0) return nf::fail(NF_ERR_INVALID, "nf_split_slab: cpd must be odd and positive");
+    if (R < 0 || R > NF_MAX_RADIUS || 2 * R + 1 > cpd)
+        return nf::fail(NF_ERR_INVALID, "nf_split_slab: nfradius out of range");
+    if (nbuffers <= 0) return nf::fail(NF_ERR_INVALID, "nf_split_slab: nbuffers must be positive");
+    const int W = 2 * R + 1;
+    for (int c = 0; c < W; c++)
+        if (!cellinfo[c]) return nf::fail(NF_ERR_INVALID, "nf_split_slab: missing cellinfo slab");
+    // NumParticlesInSkewer: next row's start (or the slab size) minus this row's.
+    auto skewer = [&](int c, int j) -> uint64_t {
+        const uint64_t start = cellinfo[c][(uint64_t)j * cpd].startindex_with_ghost;
+        const uint64_t end = j < cpd - 1 ? cellinfo[c][(uint64_t)(j + 1) * cpd].startindex_with_ghost : nslab[c];
+        return end - start;
+    };
+    std::vector<int> sink_blocks(cpd), source_blocks(cpd);
+    int64_t tot_sink = 0;
+    for (int j = 0; j < cpd; j++) {
+        const uint64_t s0 = cellinfo[R][(uint64_t)j * cpd].startindex;
+        const uint64_t s1 = j < cpd - 1 ? cellinfo[R][(uint64_t)(j + 1) * cpd].startindex : nslab[R];
+        sink_blocks[j] = W * ((int)(s1 - s0) / NF_BLOCK_SIZE) + cpd;  // ComputeSinkBlocks (no ghosts)
+        int npsrc = 0;
+        for (int c = 0; c < W; c++) npsrc += (int)skewer(c, j);
+        source_blocks[j] = npsrc / NF_BLOCK_SIZE + cpd;  // ComputeSourceBlocks
+        tot_sink += sink_blocks[j];
+    }
+    int use_max_sink = (int)(tot_sink / nbuffers);
+    use_max_sink = (int)(use_max_sink * (1 + 1.1 * nbuffers / cpd));
+    use_max_sink = std::min(use_max_sink, max_sink_blocks);
+    const int use_max_source = max_source_blocks;
+    auto wrap = [&](int j) { return ((j % cpd) + cpd) % cpd; };
+    int n = 0, this_sink = 0, this_source = 0;
+    for (int j = 0; j < cpd; j++) {
+        this_sink += sink_blocks[j];
+        this_source += source_blocks[wrap(j + R)];
+        if (j == 0 || this_sink > use_max_sink || this_source > use_max_source) {
+            if (n > 0) split[n - 1] = j;
+            this_sink = sink_blocks[j];
+            this_source = 0;
+            for (int c = -R; c <= R; c++) this_source += source_blocks[wrap(j - c)];
+            n++;
+            if (n >= 1024) return nf::fail(NF_ERR_CAPACITY, "nf_split_slab: too many splits");
+        }
+    }
+    split[n - 1] = cpd;
+    *nsplit = n;
+    if (this_sink > max_sink_blocks || this_source > max_source_blocks)
+        return nf::fail(NF_ERR_CAPACITY, "nf_split_slab: sinks or sources of the last skewer overflow the maxima");
     return NF_OK;
 }

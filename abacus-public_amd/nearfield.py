@@ -128,6 +128,8 @@ def load_library(path=LIB_PATH):
                                 C.POINTER(U64), C.c_float, C.c_float, I32, C.c_void_p, U64,
                                 C.POINTER(nf_sic)]
     lib.nf_plan_sic_device.argtypes = lib.nf_plan_sic.argtypes
+    lib.nf_split_slab.argtypes = [I32, I32, C.POINTER(C.c_void_p), C.POINTER(U64), I32, I32, I32,
+                                  C.POINTER(I32), C.POINTER(I32)]
     lib.nf_submit_slab.argtypes = [C.POINTER(nf_slab_task), I32, C.POINTER(C.c_void_p)]
     lib.nf_device_alloc.argtypes = [U64, C.POINTER(C.c_void_p)]
     lib.nf_device_free.argtypes = [C.c_void_p]
@@ -358,6 +360,7 @@ class NearFieldDriver:
         msb, mrb = C.c_int32(0), C.c_int32(0)
         check(lib().nf_setup(C.byref(p), C.byref(msb), C.byref(mrb)), "nf_setup")
         NearFieldDriver._active = self
+        self.buffers_per_device = buffers_per_device
         self.max_sink_blocks, self.max_source_blocks = msb.value, mrb.value
         self.slab_tasks = {}
 
@@ -408,13 +411,31 @@ class NearFieldDriver:
                 raise
         return task
 
+    def split_slab(self, box, slab):
+        """ExecuteSlabGPU's y-split of a slab (nf_split_slab): the end row of
+        each SIC, for this driver's worker count and block maxima."""
+        W = self.W
+        slabs = [box.slab(slab - self.R + c) for c in range(W)]
+        ci = (C.c_void_p * W)(*[s.cellinfo.ctypes.data for s in slabs])
+        nsl = (C.c_uint64 * W)(*[s.n for s in slabs])
+        out = (C.c_int32 * self.cpd)()
+        n = C.c_int32(0)
+        check(lib().nf_split_slab(self.cpd, self.R, ci, nsl, self.buffers_per_device, self.max_sink_blocks,
+                                  self.max_source_blocks, out, C.byref(n)), "nf_split_slab")
+        return [int(out[i]) for i in range(n.value)]
+
     def execute_slab(self, box, slab, acc, blocking=False, nsplit=1, planner="host"):
-        """ExecuteSlab for a host-resident box: plan the slab as ``nsplit`` SICs
-        over y (directdriver.cpp:242-391) and submit them.  ``acc`` is the
-        float32 (n, 4) acceleration array of the slab, filled on completion.
-        planner "device": each split is planned on the device (nf_submit_slab)."""
+        """ExecuteSlab for a host-resident box: plan the slab as SICs over y
+        (directdriver.cpp:242-391) and submit them.  ``nsplit``: a number of
+        equal row ranges, or "reference" for ExecuteSlabGPU's own split
+        (split_slab).  ``acc`` is the float32 (n, 4) acceleration array of the
+        slab, filled on completion.  planner "device": each split is planned on
+        the device (nf_submit_slab)."""
         cpd = self.cpd
-        bounds = np.linspace(0, cpd, nsplit + 1).round().astype(int)
+        if nsplit == "reference":
+            bounds = np.array([0] + self.split_slab(box, slab))
+        else:
+            bounds = np.linspace(0, cpd, nsplit + 1).round().astype(int)
         tasks = []
         for jl, jh in zip(bounds[:-1], bounds[1:]):
             if jh <= jl:

@@ -335,6 +335,19 @@ int nf_plan_sic(int32_t cpd, int32_t nfradius, int32_t slab, int32_t j_low, int3
                 const uint64_t *nslab, float eps, float b2, int32_t softening, void *arena,
                 uint64_t arena_bytes, nf_sic *out);
 
+/* NearFieldDriver::ExecuteSlabGPU's split of a slab into SICs over y
+ * (directdriver.cpp:248-303), with the skewer estimates ComputeSinkBlocks /
+ * ComputeSourceBlocks (PencilOnPencil.cc:113-126): a new SIC starts where the
+ * running worst-case sink or source blocks pass useMaxSink (the slab's total
+ * over nbuffers, scaled by 1 + 1.1 nbuffers/cpd, at most max_sink_blocks) or
+ * max_source_blocks.  cellinfo[c] and nslab[c] as for nf_plan_sic; nbuffers
+ * is NGPU * DirectBPD; max_*_blocks are nf_setup's.  Writes each SIC's end
+ * row into split (room for cpd entries) and their number into *nsplit;
+ * NF_ERR_CAPACITY where the reference asserts (the last skewer overflows). */
+int nf_split_slab(int32_t cpd, int32_t nfradius, const nf_cellinfo *const *cellinfo, const uint64_t *nslab,
+                  int32_t nbuffers, int32_t max_sink_blocks, int32_t max_source_blocks, int32_t *split,
+                  int32_t *nsplit);
+
 /* nf_plan_sic computed by the device planner nf_submit_slab uses (same
  * arguments, same output; cellinfo in host memory; nslab must be given, as
  * the device planner checks every cell against its slab).  Synchronous, on

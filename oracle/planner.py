@@ -109,3 +109,45 @@ def plan_sic(cpd, R, x, j_low, j_high, count, start, block=NF_BLOCK):
         SinkTotal=int(sink_count.sum()), SourceTotal=int(src_count.sum()),
         PaddedSinkTotal=nsink_blocks * block, PaddedSourceTotal=nsource_blocks * block,
     )
+
+
+def skewer_particles(count_slab, cpd):
+    """NumParticlesInSkewer (PencilOnPencil.cc:90-108): particles in each row
+    (skewer) y of a slab, the difference of consecutive rows' start indices
+    (no ghosts: startindex and startindex_with_ghost agree)."""
+    return np.asarray(count_slab, np.int64).reshape(cpd, cpd).sum(axis=1)
+
+
+def split_slab(cpd, R, counts, nbuffers, max_sink_blocks, max_source_blocks):
+    """The y-split of NearFieldDriver::ExecuteSlabGPU (directdriver.cpp:248-303):
+    per skewer the worst-case sink blocks W*(np/64) + cpd and source blocks
+    (sum over the W slabs of np)/64 + cpd (ComputeSinkBlocks/SourceBlocks,
+    PencilOnPencil.cc:113-126), then a greedy pass that starts a new SIC when
+    the running sums exceed useMaxSink or MaxSourceBlocks.  ``counts`` are the
+    cell counts of slabs x-R..x+R.  Returns the list of split points (each
+    SIC's end row) and raises ValueError where the reference asserts."""
+    W = 2 * R + 1
+    sink = [W * (int(n) // NF_BLOCK) + cpd for n in skewer_particles(counts[R], cpd)]
+    src_np = sum(skewer_particles(counts[c], cpd) for c in range(W))
+    source = [int(n) // NF_BLOCK + cpd for n in src_np]
+    use_max_sink = sum(sink) // nbuffers                 # int / int
+    use_max_sink = int(use_max_sink * (1 + 1.1 * nbuffers / cpd))  # int *= double truncates
+    use_max_sink = min(use_max_sink, max_sink_blocks)
+    use_max_source = max_source_blocks
+    split = []
+    this_sink = this_source = 0
+    for j in range(cpd):
+        this_sink += sink[j]
+        this_source += source[(j + R) % cpd]
+        if j == 0 or this_sink > use_max_sink or this_source > use_max_source:
+            if split:
+                split[-1] = j
+            this_sink = sink[j]
+            this_source = sum(source[(j - c) % cpd] for c in range(-R, R + 1))
+            split.append(None)
+            if len(split) >= 1024:
+                raise ValueError("too many splits")
+    split[-1] = cpd
+    if this_sink > max_sink_blocks or this_source > max_source_blocks:
+        raise ValueError("sinks or sources of the last skewer overflow the maxima")
+    return split

@@ -89,12 +89,19 @@ def test_batched_register_tiles_agree(box9, ref9, S):
 
 
 def test_split_sics_equal_whole_slab(box9):
-    """A slab planned as 3 SICs over y gives bitwise the same accelerations
-    as one SIC (each SIC covers whole skewers; co-add is per skewer)."""
+    """A slab planned as 3 SICs over y, or split as ExecuteSlabGPU splits it
+    (nf_split_slab, here into several SICs for 3 workers), gives bitwise the
+    same accelerations as one SIC (each SIC covers whole skewers; co-add is
+    per skewer)."""
     for path in ("plain", "batched"):
         a1 = run_box(box9, path, slabs=[2], nsplit=1)[2]
         a3 = run_box(box9, path, slabs=[2], nsplit=3)[2]
+        ar = run_box(box9, path, slabs=[2], nsplit="reference", bpd=3)[2]
         assert np.array_equal(a1, a3), path
+        assert np.array_equal(a1, ar), path
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, params(box9)[1], buffers_per_device=3,
+                             max_buffer_bytes=2 << 30) as drv:
+        assert len(drv.split_slab(box9, 2)) >= 2
 
 
 def test_deterministic(box9):

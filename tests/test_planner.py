@@ -165,3 +165,66 @@ def test_gather_restatement_layout():
             assert np.array_equal(s4[o:o + n, 0], (xyz[:, 0] + dx) if dx != 0 else xyz[:, 0])
             assert np.array_equal(s4[o:o + n, 1:3], xyz[:, 1:3])
             o += n
+
+
+def c_split(box, x, nbuffers, max_sink, max_source, R_=R):
+    import ctypes as C
+    W = 2 * R_ + 1
+    slabs = [box.slab(x - R_ + c) for c in range(W)]
+    ci = (C.c_void_p * W)(*[s.cellinfo.ctypes.data for s in slabs])
+    nsl = (C.c_uint64 * W)(*[s.n for s in slabs])
+    out = (C.c_int32 * box.cpd)()
+    n = C.c_int32(0)
+    rc = nfm.lib().nf_split_slab(box.cpd, R_, ci, nsl, nbuffers, max_sink, max_source, out, C.byref(n))
+    return rc, [int(out[i]) for i in range(n.value)]
+
+
+def uniform_cells(cpd, per_cell):
+    """A box with exactly per_cell particles in every cell."""
+    slabs = []
+    for _ in range(cpd):
+        count = np.full(cpd * cpd, per_cell, np.int32)
+        slabs.append(synth.Slab(np.zeros(3 * int(count.sum()), np.float32), count))
+    return synth.Box(cpd, slabs)
+
+
+def test_split_slab_known_answers():
+    """ExecuteSlabGPU's split worked by hand: cpd 5, R 1, 64 particles per cell.
+    Every skewer: 3*(320/64)+5 = 20 sink blocks and (3*320)/64+5 = 20 source
+    blocks; the slab's 100 sink blocks over nbuffers, scaled by 1+1.1*nb/5."""
+    box = uniform_cells(5, 64)
+    ok, big = 0, 1000
+    # nbuffers 2: useMaxSink = int(50 * 1.44) = 72 -> rows [0,3) then [3,5)
+    assert planner.split_slab(5, 1, [s.count for s in box.slabs[:3]], 2, big, big) == [3, 5]
+    assert c_split(box, 1, 2, big, big, R_=1) == (ok, [3, 5])
+    # nbuffers 1: useMaxSink = int(100 * 1.22) = 122 -> one SIC
+    assert planner.split_slab(5, 1, [s.count for s in box.slabs[:3]], 1, big, big) == [5]
+    assert c_split(box, 1, 1, big, big, R_=1) == (ok, [5])
+    # source cap 70: a skewer's 3-row source window is 60, two rows' 80 -> one row per SIC
+    assert planner.split_slab(5, 1, [s.count for s in box.slabs[:3]], 1, big, 70) == [1, 2, 3, 4, 5]
+    assert c_split(box, 1, 1, big, 70, R_=1) == (ok, [1, 2, 3, 4, 5])
+    # sink cap 15 < one skewer's 20: the reference asserts; the product returns NF_ERR_CAPACITY
+    with pytest.raises(ValueError):
+        planner.split_slab(5, 1, [s.count for s in box.slabs[:3]], 1, 15, big)
+    assert c_split(box, 1, 1, 15, big, R_=1)[0] == -3
+
+
+@pytest.mark.parametrize("name", sorted(BOXES))
+def test_split_slab_matches_oracle(name):
+    """nf_split_slab == the oracle's restatement over worker counts and block
+    maxima, and every split covers all rows in increasing order."""
+    box = BOXES[name]()
+    for x in (0, box.cpd // 2, box.cpd - 1):
+        counts = [box.slab(x - R + c).count for c in range(2 * R + 1)]
+        for nb in (1, 2, 3, 8):
+            for max_sink, max_source in ((1 << 20, 1 << 20), (400, 1 << 20), (1 << 20, 250), (300, 200)):
+                try:
+                    ref = planner.split_slab(box.cpd, R, counts, nb, max_sink, max_source)
+                except ValueError:
+                    ref = None
+                rc, got = c_split(box, x, nb, max_sink, max_source)
+                if ref is None:
+                    assert rc == -3, (name, x, nb, max_sink, max_source)
+                    continue
+                assert rc == 0 and got == ref, (name, x, nb, max_sink, max_source, got, ref)
+                assert got[-1] == box.cpd and all(a < b for a, b in zip(got, got[1:]))
