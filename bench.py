@@ -7,7 +7,10 @@ The whole box (PosXYZ slabs and float4 acceleration slabs) is resident in HBM;
 a step is one near-field evaluation of the whole box: every slab's
 SetInteractionCollection submitted through the C ABI (device-resident mode),
 gather + direct kernel + co-add on the GPU.  The SIC plans are built once from
-the cellinfo before timing (host planning time is reported separately).
+the cellinfo before timing (host planning time is reported separately); with
+--planner device every step submits the slabs through nf_submit_slab, which
+plans them on the GPU inside the timed region.  5 timed steps by default; the
+line reports their total and their median.
 
 Prints one JSON line (rank 0).  Unit of work: unpadded pair interactions
 including self pairs (DirectTotal); 20 FLOP per interaction for % of peak.
@@ -58,7 +61,7 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="uniform512", choices=sorted(WORKLOADS) + sorted(STREAM_WORKLOADS))
     ap.add_argument("--stream-slabs", type=int, default=40, help="slab-stream workloads: sink slabs per pass")
@@ -153,9 +156,11 @@ def main():
                 log(f"[bench] {path}: warmup {i + 1}/{args.warmup} done")
             nfm.device_synchronize()
             t1 = time.perf_counter()
-            kstats = []
+            kstats, step_walls = [], []
             for i in range(args.steps):
-                kstats.extend(step())
+                ts = time.perf_counter()
+                kstats.extend(step())  # returns once every task of the step has completed
+                step_walls.append(time.perf_counter() - ts)
                 log(f"[bench] {path}: step {i + 1}/{args.steps} done")
             nfm.device_synchronize()
             wall = time.perf_counter() - t1
@@ -176,6 +181,7 @@ def main():
                 e["interactions_per_launch"] = e["interactions"] / e["launches"]
             results[path] = dict(
                 wall_s=wall, ms_per_step=1e3 * wall / args.steps,
+                median_step_ms=1e3 * float(np.median(step_walls)),
                 interactions_per_s=direct_total * args.steps / wall,
                 direct_kernel_s_per_step=float(kern.sum()) / args.steps,
                 device_s_per_step=float(tot.sum()) / args.steps,
