@@ -130,14 +130,31 @@ __device__ __forceinline__ uint64_t slab_n(const DevSlabs &sl, int c) {
     return c == 0 ? sl.n[0] : c == 1 ? sl.n[1] : c == 2 ? sl.n[2] : c == 3 ? sl.n[3] : sl.n[4];
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-    for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
-    return v;
+// Wave-wide min / max (every lane of the wave active), wave-uniform result:
+// four DPP steps leave each 16-lane row's extremum in all its lanes
+// (quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror), then
+// the four rows' values are read out.  No LDS round trips, unlike a
+// __shfl_xor butterfly (ds_bpermute per step).
+template <bool MAX, int CTRL>
+__device__ __forceinline__ float dpp_step(float v) {
+    const int x = __builtin_bit_cast(int, v);
+    const float u = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false));
+    return MAX ? fmaxf(v, u) : fminf(v, u);
 }
-__device__ __forceinline__ float wave_max(float v) {
-    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-    return v;
+template <bool MAX>
+__device__ __forceinline__ float wave_reduce(float v) {
+    v = dpp_step<MAX, 0xB1>(v);   // quad_perm [1,0,3,2]
+    v = dpp_step<MAX, 0x4E>(v);   // quad_perm [2,3,0,1]
+    v = dpp_step<MAX, 0x141>(v);  // row_half_mirror
+    v = dpp_step<MAX, 0x140>(v);  // row_mirror
+    const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+    const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+    const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+    const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+    return MAX ? fmaxf(fmaxf(r0, r1), fmaxf(r2, r3)) : fminf(fminf(r0, r1), fminf(r2, r3));
 }
+__device__ __forceinline__ float wave_min(float v) { return wave_reduce<false>(v); }
+__device__ __forceinline__ float wave_max(float v) { return wave_reduce<true>(v); }
 
 // One wave per kGatherBlocks padded 64-source blocks; it writes each block
 // and its bounding box (padding excluded): box[2b] = min, box[2b+1] = max.
