@@ -3,6 +3,8 @@
 Every test calls the product through the C ABI (libnfdirect.so via
 nearfield.py) and checks it against oracle/ on the same seeded inputs.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -13,6 +15,7 @@ from nfhelpers import (R, not_soft_dominated, oracle_slab, params, rel_err, run_
                        summarize)
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def run_box(box, path, softening="single_spline", slabs=None, nsplit=1, sinks_per_lane=0, bpd=2):
@@ -77,6 +80,26 @@ def test_plain_and_batched_vs_fp64(box9, ref9):
             assert st["rms"] < lim_rms and st["max"] < lim_max, (p, x, st)
             # FOF density w (self term b2 included) against the FP64 sum
             np.testing.assert_allclose(acc[:, 3], ref[:, 3], rtol=2e-6, atol=1e-6 * float(b2))
+
+
+def test_reference_ewald_positions():
+    """The particle positions of the reference's own force-accuracy test
+    (Tests/Ewald, 65,536 particles, tests/golden/ewald_pos_f32.npy) at CPD 11
+    (~49 per cell): both paths against the FP64 oracle, at the same bars as
+    the synthetic boxes."""
+    pos = np.load(os.path.join(GOLDEN, "ewald_pos_f32.npy")).astype(np.float64)
+    box = synth.box_from_global(pos, 11, name="ewald")
+    eps, b2, _ = params(box)
+    slabs = [0, 5, 10]
+    res = {p: run_box(box, p, slabs=slabs) for p in ("plain", "batched")}
+    for x in slabs:
+        ref, soft = oracle_slab(box, x, "single_spline", eps, b2, with_soft=True)
+        keep = not_soft_dominated(ref, soft)
+        for p in ("plain", "batched"):
+            assert np.isfinite(res[p][x]).all(), p
+            st = summarize(rel_err(res[p][x], ref), keep)
+            lim_rms, lim_max = (3e-6, 1e-4) if p == "plain" else (1e-6, 1e-5)
+            assert st["rms"] < lim_rms and st["max"] < lim_max, (p, x, st)
 
 
 @pytest.mark.parametrize("S", [1, 2, 4])

@@ -48,3 +48,24 @@ def test_box_from_global_orders_cells():
                             synth.cell_centre(cz, 9)], axis=1)
         cell = np.floor((g + 0.5) * 9).astype(int)
         assert (cell[:, 0] == x).all() and (cell[:, 1] == cy).all() and (cell[:, 2] == cz).all()
+
+
+def test_reference_ewald_fixture():
+    """tests/golden/ewald_pos_f32.npy: the 65,536 positions of the reference's
+    Ewald test, inside the unit box, binned into a box the planner accepts;
+    the planner's DirectTotal over all slabs equals the brute-force count
+    (CalcDI) of the oracle."""
+    import os
+    pos = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ewald_pos_f32.npy"))
+    assert pos.shape == (65536, 3) and pos.dtype == np.float32
+    assert np.all(pos >= -0.5) and np.all(pos < 0.5)
+    box = synth.box_from_global(pos.astype(np.float64), 11)
+    assert box.N == 65536 and box.counts().sum() == 65536
+    import nearfield as nfm
+    import pyoracle as po
+    from nfhelpers import R, params
+    eps, b2, _ = params(box)
+    for x in (0, 5, 10):
+        sic = nfm.SetInteractionCollection(box, x, 0, box.cpd, R, eps, b2, "single_spline")
+        assert sic.desc.DirectTotal == po.calc_di(box, R, x)
+
