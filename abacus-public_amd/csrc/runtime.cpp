@@ -189,6 +189,15 @@ struct Runtime {
         int uses = 0;
     };
     std::map<std::string, TuneEntry> tune;  // occupancy-histogram key -> tuned shape
+    // The GPU throughput timer (GPUControl.cu:658-672): wall time during which
+    // at least one launch group is on the GPU, and totals of completed tasks.
+    struct Throughput {
+        std::mutex m;
+        int active = 0;
+        std::chrono::steady_clock::time_point since;
+        double active_s = 0.0;
+        uint64_t direct = 0, padded = 0, to_dev = 0, from_dev = 0, tasks = 0, groups = 0;
+    } tp;
     hipStream_t shared = nullptr;  // when gpu_streams == 1: the one stream all workers use
     std::mutex enqueue;            // orders whole task sequences on the shared stream
 
@@ -712,6 +721,25 @@ void release_slab(Runtime &rt, Runtime::Slot *sl) {
     if (--sl->refs == 0 && !(C.active && sl->epoch == C.epoch)) sl->valid = false;
 }
 
+// Counts a launch group as on the GPU from its first enqueue to its
+// completion, for the throughput timer.
+struct ActiveOnGpu {
+    Runtime::Throughput &t;
+    bool on = false;
+    void start() {
+        std::lock_guard<std::mutex> lk(t.m);
+        if (t.active++ == 0) t.since = clk::now();
+        on = true;
+    }
+    void stop() {
+        if (!on) return;
+        on = false;
+        std::lock_guard<std::mutex> lk(t.m);
+        if (--t.active == 0) t.active_s += std::chrono::duration<double>(clk::now() - t.since).count();
+    }
+    ~ActiveOnGpu() { stop(); }
+};
+
 // Drops the slab-cache references a group holds when it is done.
 struct SlotRelease {
     Runtime &rt;
@@ -1137,6 +1165,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     for (size_t k = 0; k < hslot.size(); k++) NF_HIP(hipStreamWaitEvent(stream, hslot[k]->ready, 0));
     for (size_t k = 0; k < hslab.size(); k++)
         if (fresh[k]) to_dev += 12 * hslab_n[k];
+    ActiveOnGpu on_gpu{rt.tp};
+    on_gpu.start();
     NF_HIP(hipEventRecord(w.ev[1], stream));
     // The captured region holds kernels only; copies and timing events stay
     // outside it, around the graph launch.
@@ -1217,6 +1247,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     NF_HIP(hipEventRecord(w.ev[4], stream));
     if (enqueue_lock.owns_lock()) enqueue_lock.unlock();
     NF_HIP(hipEventSynchronize(w.ev[4]));
+    on_gpu.stop();
     const double kern_ms = event_ms(w.ev[1], w.ev[4]);
     // A graph is timed as a whole (gathers and co-adds included).
     const double direct_ms = use_graph ? kern_ms : event_ms(w.ev[2], w.ev[3]);
@@ -1304,6 +1335,17 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         st.waves_per_eu = path == NF_PATH_BATCHED ? batched_kernel_waves(s0.softening, ts.sinks_per_lane) : 0;
         st.device_planned = g[i]->dp.on ? 1 : 0;
         st.plan_s = g[i]->dp.on ? std::chrono::duration<double>(t_plan - t0).count() : 0.0;
+    }
+    {
+        std::lock_guard<std::mutex> lk(rt.tp.m);
+        for (int i = 0; i < n; i++) {
+            rt.tp.direct += g[i]->sic.DirectTotal;
+            rt.tp.padded += g[i]->sic.PaddedDirectTotal;
+        }
+        rt.tp.to_dev += to_dev;
+        rt.tp.from_dev += from_dev;
+        rt.tp.tasks += (uint64_t)n;
+        rt.tp.groups++;
     }
     return NF_OK;
 }
@@ -1816,6 +1858,22 @@ extern "C" int nf_host_alloc(uint64_t bytes, void **ptr) {
 
 extern "C" int nf_host_free(void *ptr) {
     if (ptr) NF_HIP(hipHostFree(ptr));
+    return NF_OK;
+}
+
+extern "C" int nf_get_throughput(nf_throughput *out) {
+    if (!out) return fail(NF_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(g_rt_mutex);
+    if (!g_rt) return fail(NF_ERR_STATE, "nf_get_throughput before nf_setup");
+    auto &t = g_rt->tp;
+    std::lock_guard<std::mutex> tl(t.m);
+    out->gpu_active_s = t.active_s + (t.active ? std::chrono::duration<double>(clk::now() - t.since).count() : 0.0);
+    out->direct_total = t.direct;
+    out->padded_direct_total = t.padded;
+    out->bytes_to_device = t.to_dev;
+    out->bytes_from_device = t.from_dev;
+    out->tasks = t.tasks;
+    out->groups = t.groups;
     return NF_OK;
 }
 

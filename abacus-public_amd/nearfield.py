@@ -87,6 +87,12 @@ class nf_task_stats(C.Structure):
     ]
 
 
+class nf_throughput(C.Structure):
+    _fields_ = [("gpu_active_s", C.c_double), ("direct_total", C.c_uint64), ("padded_direct_total", C.c_uint64),
+                ("bytes_to_device", C.c_uint64), ("bytes_from_device", C.c_uint64), ("tasks", C.c_uint64),
+                ("groups", C.c_uint64)]
+
+
 class nf_slab_task(C.Structure):
     _fields_ = [
         ("SlabId", C.c_int32), ("j_low", C.c_int32), ("j_high", C.c_int32), ("cpd", C.c_int32),
@@ -138,6 +144,7 @@ def load_library(path=LIB_PATH):
     lib.nf_host_alloc.argtypes = [U64, C.POINTER(C.c_void_p)]
     lib.nf_host_free.argtypes = [C.c_void_p]
     lib.nf_autotune_table.argtypes = [C.c_char_p, C.c_size_t]
+    lib.nf_get_throughput.argtypes = [C.POINTER(nf_throughput)]
     lib.nf_stream_begin.argtypes = []
     lib.nf_stream_end.argtypes = []
     return lib
@@ -467,6 +474,15 @@ class NearFieldDriver:
 
     def stream_end(self):
         check(lib().nf_stream_end(), "nf_stream_end")
+
+    def throughput(self):
+        """The GPU throughput counters since setup, with the reference's
+        "effective GDIPS" (giga direct interactions per GPU-active second)."""
+        t = nf_throughput()
+        check(lib().nf_get_throughput(C.byref(t)), "nf_get_throughput")
+        d = {k: getattr(t, k) for k, _ in t._fields_}
+        d["gdips"] = d["direct_total"] / d["gpu_active_s"] / 1e9 if d["gpu_active_s"] > 0 else 0.0
+        return d
 
     def autotune_table(self):
         """The autotuner's table: occupancy histogram -> chosen tile shape."""

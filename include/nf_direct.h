@@ -276,6 +276,19 @@ int nf_release(nf_task *task);
 /* Thread-local description of the last error. */
 const char *nf_last_error(void);
 
+/* The GPU throughput timer and totals (GPUControl.cu:658-672, read by
+ * NearFieldDriver::AggregateStats, directdriver.cpp:495-520): wall seconds
+ * during which at least one launch group was on the GPU, and totals of the
+ * tasks completed since nf_setup.  direct_total / gpu_active_s is the
+ * reference's "effective GDIPS" (reporting.cpp:233-236). */
+typedef struct nf_throughput {
+    double gpu_active_s;
+    uint64_t direct_total, padded_direct_total;
+    uint64_t bytes_to_device, bytes_from_device;
+    uint64_t tasks, groups;
+} nf_throughput;
+int nf_get_throughput(nf_throughput *out);
+
 /* The batched-path autotuner's table (no counterpart in the reference).  With
  * sinks_per_lane = 0 the runtime keys every launch group on a coarse histogram
  * of its sink-pencil occupancies (the share of its sinks in pencils of <64,

@@ -43,6 +43,10 @@ def test_struct_layout_matches_c(tmp_path):
     for f in slab_fields:
         src.append(f'printf("%zu\\n", offsetof(nf_slab_task, {f}));')
     src.append('printf("%zu\\n", sizeof(nf_slab_task));')
+    tp_fields = [f for f, _ in nfm.nf_throughput._fields_]
+    for f in tp_fields:
+        src.append(f'printf("%zu\\n", offsetof(nf_throughput, {f}));')
+    src.append('printf("%zu\\n", sizeof(nf_throughput));')
     src.append('printf("%zu\\n", offsetof(nf_kick, NearAccOut));')
     src.append('printf("%zu %zu %zu\\n", offsetof(nf_kick, density_rad2), sizeof(nf_kick), sizeof(nf_setup_params));'
                ' return 0; }')
@@ -61,6 +65,9 @@ def test_struct_layout_matches_c(tmp_path):
     k = 5 + len(fields)
     offs = list(map(int, out[k:k + len(slab_fields) + 1]))
     assert offs == [getattr(nfm.nf_slab_task, f).offset for f in slab_fields] + [C.sizeof(nfm.nf_slab_task)]
+    k += len(slab_fields) + 1
+    offs = list(map(int, out[k:k + len(tp_fields) + 1]))
+    assert offs == [getattr(nfm.nf_throughput, f).offset for f in tp_fields] + [C.sizeof(nfm.nf_throughput)]
     assert list(map(int, out[-4:])) == [nfm.nf_kick.NearAccOut.offset, nfm.nf_kick.density_rad2.offset,
                                         C.sizeof(nfm.nf_kick), C.sizeof(nfm.nf_setup_params)]
 
@@ -99,6 +106,7 @@ def test_runtime_errors_without_setup():
     assert lib.nf_submit(C.byref(nfm.nf_sic()), 0, C.byref(h)) == -4
     assert lib.nf_submit_slab(C.byref(nfm.nf_slab_task()), 0, C.byref(h)) == -4
     assert lib.nf_submit_slab(None, 0, C.byref(h)) == -1
+    assert lib.nf_get_throughput(C.byref(nfm.nf_throughput())) == -4
     assert lib.nf_reset() == -4
     assert lib.nf_wait(None) == -1
     bad = nfm.nf_setup_params(cpd=7, max_buffer_bytes=1 << 30, n_gpu=2, buffers_per_device=1, path=1)

@@ -414,3 +414,56 @@ def test_fused_kick_matches_consumer(box9, path, mem):
         assert np.array_equal(fused2[x], expect), x
         assert np.array_equal(record[x], po.near_acc_record(near[x], inv_eps3)), x
         assert np.abs(fused[x][:, 3]).max() < 10 * dens      # w minus its self term is small
+
+
+def test_throughput_counters(box9):
+    """nf_get_throughput: the GPU-active wall time and the totals of every
+    completed task since setup (the reference's GPU throughput timer and
+    AggregateStats totals)."""
+    eps, b2, _ = params(box9)
+    slabs = list(range(box9.cpd))
+    with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=2,
+                             max_buffer_bytes=2 << 30, sinks_per_lane=2) as drv:
+        assert drv.throughput()["tasks"] == 0
+        import time
+        t0 = time.perf_counter()
+        accs = {x: np.full((box9.slab(x).n, 4), np.nan, np.float32) for x in slabs}
+        for x in slabs:
+            drv.execute_slab(box9, x, accs[x], nsplit=2)
+        stats = [st for x in slabs for st in drv.finalize(x)]
+        wall = time.perf_counter() - t0
+        tp = drv.throughput()
+    assert tp["tasks"] == len(stats) == 2 * box9.cpd
+    assert 1 <= tp["groups"] <= tp["tasks"]
+    assert tp["direct_total"] == sum(st["direct_total"] for st in stats)
+    assert tp["padded_direct_total"] == sum(st["padded_direct_total"] for st in stats)
+    assert 0 < tp["gpu_active_s"] <= wall and tp["gdips"] > 0
+    assert abs(tp["bytes_to_device"] - sum(st["bytes_to_device"] for st in stats)) <= tp["tasks"]
+
+
+def test_oversized_sic_fails_with_capacity_error():
+    """A SIC whose working set exceeds the buffer fails with NF_ERR_CAPACITY
+    (the reference asserts), the driver keeps working, and the same slab as
+    one SIC per row succeeds with bitwise the accelerations of a whole-slab
+    SIC on a driver with room."""
+    box = synth.uniform_box(33 ** 3 * 300, 33, seed=31)  # ~107 MB working set per whole-slab SIC
+    eps, b2, _ = params(box)
+    x = 16
+    small = 64 * (1 << 20) + (2 << 20)  # nf_setup keeps 64 MiB of headroom; ~28 MB per one-row SIC
+    with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                             max_buffer_bytes=small, sinks_per_lane=2) as drv:
+        acc = np.full((box.slab(x).n, 4), np.nan, np.float32)
+        with pytest.raises(nfm.NFError) as ei:
+            drv.execute_slab(box, x, acc, blocking=True)
+        assert "device bytes" in str(ei.value) and "(-3)" in str(ei.value)
+        drv.slab_tasks.pop(x % box.cpd, None)
+        rows = np.full_like(acc, np.nan)
+        drv.execute_slab(box, x, rows, nsplit=box.cpd)
+        drv.finalize(x)
+    with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                             max_buffer_bytes=2 << 30, sinks_per_lane=2) as drv:
+        whole = np.full_like(acc, np.nan)
+        drv.execute_slab(box, x, whole, blocking=True)
+        drv.finalize(x)
+    assert np.isfinite(whole).all() and np.array_equal(rows, whole)
+
