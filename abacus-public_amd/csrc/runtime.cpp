@@ -9,6 +9,7 @@
 // Completion is an atomic flag the caller polls (CheckCompletion).
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <atomic>
@@ -721,6 +722,13 @@ void release_slab(Runtime &rt, Runtime::Slot *sl) {
     if (--sl->refs == 0 && !(C.active && sl->epoch == C.epoch)) sl->valid = false;
 }
 
+// A roctx range around a launch group's work on its worker (visible with
+// rocprofv3 --marker-trace; a no-op without a tool attached).
+struct TraceRange {
+    explicit TraceRange(const char *msg) { roctxRangePushA(msg); }
+    ~TraceRange() { roctxRangePop(); }
+};
+
 // Counts a launch group as on the GPU from its first enqueue to its
 // completion, for the throughput timer.
 struct ActiveOnGpu {
@@ -825,6 +833,10 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     const int path = rt.params.path;
     TileShape ts = rt.shape;
     const int n = (int)g.size();
+    char label[96];
+    std::snprintf(label, sizeof label, "nf group: %d SIC%s from slab %d rows [%d,%d)%s", n, n > 1 ? "s" : "",
+                  g[0]->sic.SlabId, g[0]->sic.j_low, g[0]->sic.j_high, g[0]->dp.on ? ", device-planned" : "");
+    TraceRange trace_range(label);
     // Tasks from nf_submit_slab (grouped only with each other) are planned
     // on the device: phase A first, since everything below needs the sizes.
     const bool dplan = g[0]->dp.on;
