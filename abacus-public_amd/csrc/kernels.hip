@@ -4,6 +4,8 @@
 //       PosXYZ slabs into padded float4 pencil streams (the device-side form of
 //       SinkPencilPlan/SourcePencilPlan::copy_into_pinned_memory,
 //       PencilPlan.cc:22-90, 214-255), plus a bounding box per 64-source block.
+//   k_gather_sources_binned  the batched path's source stream: each cell's
+//       particles in x-bin order, FP64 cell offsets, block boxes.
 //   k_plain      one 64-sink block per workgroup, one launch per SIC: the
 //       structure of ComputeDirects (PencilKernel.cu:97-189) with the
 //       reference's pair forms (DirectGPUKernels.cuh) and BLOCK_ACCUM tiles.
@@ -74,10 +76,8 @@ __device__ __forceinline__ float3 slab_xyz(const float *P, uint64_t n, uint64_t 
 // chains in flight.
 constexpr int kGatherBlocks = 4;
 
-template <bool F64>
 __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *__restrict__ slab, uint64_t n,
-                                                      int aos, float4 *__restrict__ out, float cellsize,
-                                                      double invcpd) {
+                                                      int aos, float4 *__restrict__ out, float cellsize) {
     const int b0 = (blockIdx.x * 4 + (int)(threadIdx.x / kWave)) * kGatherBlocks;
     if (b0 >= pl.nsink_blocks) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -111,12 +111,8 @@ __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *_
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live[u]) {
             float z = q[u].z;
-            if (F64) {
-                z = (float)((double)z + (double)dcell[u] * invcpd);
-            } else {  // PencilPlan.cc:43-50: dz = (c-R)*cellsize in FLOAT, added if nonzero
-                const float dz = (float)dcell[u] * cellsize;
-                if (dz != 0.f) z = z + dz;
-            }
+            const float dz = (float)dcell[u] * cellsize;  // PencilPlan.cc:43-50: FLOAT, added if nonzero
+            if (dz != 0.f) z = z + dz;
             v = make_float4(q[u].x, q[u].y, z, 0.f);
         }
         out[(int64_t)(b0 + u) * kWave + lane] = v;
@@ -156,12 +152,11 @@ __device__ __forceinline__ float wave_reduce(float v) {
 __device__ __forceinline__ float wave_min(float v) { return wave_reduce<false>(v); }
 __device__ __forceinline__ float wave_max(float v) { return wave_reduce<true>(v); }
 
-// One wave per kGatherBlocks padded 64-source blocks; it writes each block
-// and its bounding box (padding excluded): box[2b] = min, box[2b+1] = max.
-template <bool F64>
+// Plain-path source stream, in the reference's order.  One wave per
+// kGatherBlocks padded 64-source blocks; it writes each block and its
+// bounding box (padding excluded): box[2b] = min, box[2b+1] = max.
 __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
-                                                        float4 *__restrict__ box, float cellsize,
-                                                        double invcpd) {
+                                                        float4 *__restrict__ box, float cellsize) {
     const int b0 = (blockIdx.x * 4 + (int)(threadIdx.x / kWave)) * kGatherBlocks;
     if (b0 >= pl.nsrc_blocks) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -200,13 +195,8 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
         const bool real = live[u];
         if (real) {
             float x = q[u].x;
-            const int dc = cc[u] - pl.R;
-            if (F64) {
-                x = (float)((double)x + (double)dc * invcpd);
-            } else {  // PencilPlan.cc:226-238
-                const float dx = (float)dc * cellsize;
-                if (dx != 0.f) x = x + dx;
-            }
+            const float dx = (float)(cc[u] - pl.R) * cellsize;  // PencilPlan.cc:226-238
+            if (dx != 0.f) x = x + dx;
             v = make_float4(x, q[u].y, q[u].z, 0.f);
         }
         out[(int64_t)b * kWave + lane] = v;
@@ -218,6 +208,110 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
             box[2 * (int64_t)b] = make_float4(mnx, mny, mnz, 0.f);
             box[2 * (int64_t)b + 1] = make_float4(mxx, mxy, mxz, 0.f);
         }
+    }
+}
+
+// Batched-path source stream.  Each cell's particles are ordered by
+// kXBins equal x-bins of the cell (a stable counting sort: within a bin
+// the reference's order), so a 64-source block that straddles two cells
+// takes an x-slice of each rather than all of one: tighter block boxes, so
+// more sink-block/source-block pairs clear the softening and density radii
+// and run the fast pair form.  Cells above kBinnedCellMax = 128 particles
+// keep the reference's order: ranking one costs a pass over the whole cell
+// per block it overlaps, and a replay of the mode split on a clustered box
+// gained little from sorting them.
+// Cell offsets are added in FP64 (x + dc/cpd, rounded once).
+//
+// One wave per block: for each cell overlapping the block's 64 slots the
+// wave ranks the whole cell (ballots per bin) and the lanes whose rank falls
+// in the block write their source there; padding slots take kFar.  The box
+// is reduced over what the wave wrote.  All of the block's binned cells
+// (two 64-particle chunks each, in registers) load before any is ranked, so
+// a block costs one load latency.  (One block per wave measured faster than
+// 2, 4 or 8 walked in turn: more waves in flight.)
+constexpr int kXBins = 8;
+constexpr int kBinnedCellMax = 2 * kWave;
+
+__device__ __forceinline__ int xbin(float x, float cpdf) {  // x cell-centred; out-of-cell x clamps
+    const float t = floorf((x * cpdf + 0.5f) * (float)kXBins);
+    return (int)fminf(fmaxf(t, 0.f), (float)(kXBins - 1));
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ int lanes_below(uint64_t m) {  // set bits of m below this lane
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
+                                                               float4 *__restrict__ box, float cpdf,
+                                                               double invcpd) {
+    const int b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
+    if (b >= pl.nsrc_blocks) return;
+    const int s = cptr<int32_t>(pl.src_parent)[b];
+    if (s < 0) return;  // a block no pencil covers is never read: left unwritten
+    const int lane = threadIdx.x & (kWave - 1);
+    const int W = pl.W;
+    const float inf = __builtin_huge_valf();
+    const nf_source_plan p = cptr<nf_source_plan>(pl.src_plan)[s];
+    const int l0 = b * kWave - cptr<int32_t>(pl.src_set_start)[s];  // the block's first slot in its pencil
+    int pre[NF_MAX_WIDTH + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int c = 0; c < NF_MAX_WIDTH; c++) pre[c + 1] = pre[c] + (c < W ? p.cell[c].N : 0);
+    float4 *blk = out + (int64_t)b * kWave;
+    if (l0 + lane >= pre[NF_MAX_WIDTH]) blk[lane] = make_float4(kFar, 0.f, 0.f, 0.f);  // cells >= W hold 0
+    float mnx = inf, mny = inf, mnz = inf, mxx = -inf, mxy = -inf, mxz = -inf;
+    auto put = [&](int slot, float3 q, int c) {
+        const float x = (float)((double)q.x + (double)(c - pl.R) * invcpd);
+        blk[slot] = make_float4(x, q.y, q.z, 0.f);
+        mnx = fminf(mnx, x), mny = fminf(mny, q.y), mnz = fminf(mnz, q.z);
+        mxx = fmaxf(mxx, x), mxy = fmaxf(mxy, q.y), mxz = fmaxf(mxz, q.z);
+    };
+    auto overlaps = [&](int c) { return c < W && pre[c + 1] > pre[c] && pre[c + 1] > l0 && pre[c] < l0 + kWave; };
+    // the binned cells' particles (two chunks each) all load before any is ranked
+    float3 q[NF_MAX_WIDTH][2];
+#pragma unroll
+    for (int c = 0; c < NF_MAX_WIDTH; c++) {
+        const int n = pre[c + 1] - pre[c];
+        q[c][0] = q[c][1] = make_float3(0.f, 0.f, 0.f);
+        if (overlaps(c) && n <= kBinnedCellMax) {
+            const float *P = slab_ptr(sl, c);
+            const uint64_t ns = slab_n(sl, c), first = p.cell[c].start;
+            if (lane < n) q[c][0] = slab_xyz(P, ns, first + (uint64_t)lane, sl.aos);
+            if (lane + kWave < n) q[c][1] = slab_xyz(P, ns, first + (uint64_t)(lane + kWave), sl.aos);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NF_MAX_WIDTH; c++) {
+        if (!overlaps(c)) continue;
+        const int n = pre[c + 1] - pre[c];
+        const int r0 = l0 - pre[c];  // the block's first slot as a rank in cell c (may be < 0)
+        if (n <= kBinnedCellMax) {
+            const int bin0 = lane < n ? xbin(q[c][0].x, cpdf) : -1;
+            const int bin1 = lane + kWave < n ? xbin(q[c][1].x, cpdf) : -1;
+            int rank0 = 0, rank1 = 0, acc = 0;
+#pragma unroll
+            for (int k = 0; k < kXBins; k++) {  // within a bin, chunk 0 ranks before chunk 1
+                const uint64_t m0 = ballot(bin0 == k);
+                if (bin0 == k) rank0 = acc + lanes_below(m0);
+                acc += __builtin_popcountll(m0);
+                if (n > kWave) {
+                    const uint64_t m1 = ballot(bin1 == k);
+                    if (bin1 == k) rank1 = acc + lanes_below(m1);
+                    acc += __builtin_popcountll(m1);
+                }
+            }
+            if (bin0 >= 0 && rank0 - r0 >= 0 && rank0 - r0 < kWave) put(rank0 - r0, q[c][0], c);
+            if (bin1 >= 0 && rank1 - r0 >= 0 && rank1 - r0 < kWave) put(rank1 - r0, q[c][1], c);
+            continue;
+        }
+        const int i = r0 + lane;  // a large cell: the reference's order
+        if (i >= 0 && i < n) put(lane, slab_xyz(slab_ptr(sl, c), slab_n(sl, c), p.cell[c].start + (uint64_t)i, sl.aos), c);
+    }
+    mnx = wave_min(mnx), mny = wave_min(mny), mnz = wave_min(mnz);
+    mxx = wave_max(mxx), mxy = wave_max(mxy), mxz = wave_max(mxz);
+    if (lane == 0) {
+        box[2 * (int64_t)b] = make_float4(mnx, mny, mnz, 0.f);
+        box[2 * (int64_t)b + 1] = make_float4(mxx, mxy, mxz, 0.f);
     }
 }
 
@@ -852,32 +946,28 @@ int device_cu_count() {
     return n;
 }
 
-int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void *sinks4, void *sources4,
+int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool batched_stream, void *sinks4, void *sources4,
                   void *stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     const float cellsize = (float)(1.0 / (double)pl.cpd);
     const double invcpd = 1.0 / (double)pl.cpd;
     float4 *box = static_cast<float4 *>(box_after(sources4, pl.nsrc_blocks));
-    auto grid_of = [](int blocks) {  // 4 waves per workgroup, kGatherBlocks blocks per wave
-        return dim3((unsigned)((blocks + 4 * kGatherBlocks - 1) / (4 * kGatherBlocks)));
+    auto grid_of = [](int blocks, int per_wave = kGatherBlocks) {  // 4 waves per workgroup
+        return dim3((unsigned)((blocks + 4 * per_wave - 1) / (4 * per_wave)));
     };
     if (sinks4 && pl.nsink_blocks > 0) {
         const dim3 grid = grid_of(pl.nsink_blocks);
-        if (fp64_offsets)
-            hipLaunchKernelGGL(k_gather_sinks<true>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R], sl.aos,
-                               static_cast<float4 *>(sinks4), cellsize, invcpd);
-        else
-            hipLaunchKernelGGL(k_gather_sinks<false>, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R], sl.aos,
-                               static_cast<float4 *>(sinks4), cellsize, invcpd);
+        hipLaunchKernelGGL(k_gather_sinks, grid, dim3(256), 0, st, pl, sl.pos[pl.R], sl.n[pl.R], sl.aos,
+                           static_cast<float4 *>(sinks4), cellsize);
     }
     if (pl.nsrc_blocks > 0) {
         const dim3 grid = grid_of(pl.nsrc_blocks);
-        if (fp64_offsets)
-            hipLaunchKernelGGL(k_gather_sources<true>, grid, dim3(256), 0, st, pl, sl, static_cast<float4 *>(sources4),
-                               box, cellsize, invcpd);
+        if (batched_stream)
+            hipLaunchKernelGGL(k_gather_sources_binned, grid_of(pl.nsrc_blocks, 1), dim3(256), 0, st, pl, sl,
+                               static_cast<float4 *>(sources4), box, (float)pl.cpd, invcpd);
         else
-            hipLaunchKernelGGL(k_gather_sources<false>, grid, dim3(256), 0, st, pl, sl,
-                               static_cast<float4 *>(sources4), box, cellsize, invcpd);
+            hipLaunchKernelGGL(k_gather_sources, grid, dim3(256), 0, st, pl, sl, static_cast<float4 *>(sources4),
+                               box, cellsize);
     }
     return (int)hipGetLastError();
 }

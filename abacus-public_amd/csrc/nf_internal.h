@@ -108,8 +108,11 @@ struct TileShape {
 // Kernel launchers (kernels.hip).  All enqueue on `stream` and return a HIP
 // error code (0 on success).
 // Source pencils always; sink pencils only if sinks4 is non-null (the plain
-// path; the batched kernel reads its sinks from the slab itself).
-int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool fp64_offsets, void *sinks4,
+// path; the batched kernel reads its sinks from the slab itself).  With
+// batched_stream the sources take FP64 cell offsets and each cell's particles
+// are ordered by x-bin (tighter block boxes); otherwise the reference's
+// order and FLOAT offsets.
+int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool batched_stream, void *sinks4,
                   void *sources4, void *stream);
 int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const void *sinks4,
                  const void *sources4, void *partial4, void *stream);

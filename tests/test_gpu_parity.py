@@ -164,6 +164,45 @@ def test_empty_and_heavy_cells():
             assert st["rms"] < 3e-6, (path, x, st)
 
 
+def test_batched_source_order_regimes():
+    """The batched path orders each source cell's particles by x-bin: cells of
+    up to 64 particles (one chunk), 65-128 (two chunks), and larger cells kept
+    in the reference's order.  Cells of 64, 65, 100, 128, 129 and 300
+    particles, some particles exactly on bin edges and some just outside
+    their cell: every source must land in the stream exactly once, so both
+    paths meet the FP64 oracle at the bars of the uniform boxes."""
+    cpd = 7
+    rng = np.random.default_rng(21)
+    w = 1.0 / cpd
+    centre = lambda i: (np.asarray(i) + 0.5) * w - 0.5
+    extra = {(2, 1, 1): 64, (2, 4, 5): 65, (3, 2, 2): 100, (3, 5, 0): 128, (4, 3, 3): 129, (3, 0, 6): 300}
+    bg = rng.random((cpd ** 3 * 30, 3)) - 0.5
+    bg_cell = np.floor((bg + 0.5) * cpd).astype(int)
+    pts = [bg[~np.isin(bg_cell @ [cpd * cpd, cpd, 1], [x * cpd * cpd + y * cpd + z for x, y, z in extra])]]
+    for cell, n in extra.items():
+        p = centre(cell) + (rng.random((n, 3)) - 0.5) * w * 0.999
+        k = np.arange(n // 8) % 7 + 1
+        p[: n // 8, 0] = centre(cell)[0] + (k / 8.0 - 0.5) * w  # on the interior bin edges
+        pts.append(p)
+    box = synth.box_from_global(np.concatenate(pts), cpd)
+    for (x, y, z), n in extra.items():
+        assert box.slab(x).count[y * cpd + z] == n, (x, y, z)
+    # cell-local x just past the cell edge (a particle that has drifted): the bins clamp
+    s3 = box.slab(3)
+    s3.pos[:5] = np.float32(0.5 * w * 1.01)
+    eps, b2, _ = params(box)
+    slabs = [2, 3, 4]
+    res = {p: run_box(box, p, slabs=slabs) for p in ("plain", "batched")}
+    for x in slabs:
+        ref, soft = oracle_slab(box, x, "single_spline", eps, b2, with_soft=True)
+        keep = not_soft_dominated(ref, soft)
+        for p in ("plain", "batched"):
+            assert np.isfinite(res[p][x]).all(), p
+            st = summarize(rel_err(res[p][x], ref), keep)
+            lim_rms, lim_max = (3e-6, 1e-4) if p == "plain" else (1e-6, 1e-5)
+            assert st["rms"] < lim_rms and st["max"] < lim_max, (p, x, st)
+
+
 def test_minimum_cpd_wraps():
     """cpd = 2R+1: every pencil wraps, every source slab is distinct."""
     box = synth.uniform_box(5 ** 3 * 12, 5, seed=9)
