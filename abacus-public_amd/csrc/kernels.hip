@@ -624,8 +624,10 @@ __device__ __forceinline__ int tile_mode(const BatchCtx &cx, float gap2) {
 // terms of each batch of 8 sources are summed on their own before they join
 // the tile sum: a sum of 8 batch sums of 8 terms carries about 1.6x less
 // FP32 rounding than one running sum of 64 terms, for one add per component
-// per batch.
-template <int SOFT, int MODE, int S>
+// per batch.  Sink slots u < S/2 take pair form MLO, the others MHI (S = 1:
+// MHI); every form gives the same value for the pairs its box test admits,
+// so the split changes which code runs, not the result.
+template <int SOFT, int MLO, int MHI, int S>
 __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, const float *kx, const float *ky,
                                           const float *kz, const PairConst &pc, float *tx, float *ty, float *tz,
                                           float *tw) {
@@ -637,8 +639,11 @@ __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, c
         for (int q = 0; q < 8; q++) {
             const float4 sv = sp[i + q];
 #pragma unroll
-            for (int u = 0; u < S; u++)
-                pair_box<SOFT, MODE>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, bx[u], by[u], bz[u], tw[u]);
+            for (int u = 0; u < S / 2; u++)
+                pair_box<SOFT, MLO>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, bx[u], by[u], bz[u], tw[u]);
+#pragma unroll
+            for (int u = S / 2; u < S; u++)
+                pair_box<SOFT, MHI>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, bx[u], by[u], bz[u], tw[u]);
         }
 #pragma unroll
         for (int u = 0; u < S; u++) {
@@ -646,6 +651,27 @@ __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, c
             ty[u] += by[u];
             tz[u] += bz[u];
         }
+    }
+}
+
+// A tile with the modes of its two halves of sink slots.
+template <int SOFT, int S>
+__device__ __forceinline__ void mixed_tile(int mlo, int mhi, const NF_CONST float4 *sp, int cnt8, const float *kx,
+                                           const float *ky, const float *kz, const PairConst &pc, float *tx,
+                                           float *ty, float *tz, float *tw) {
+    if (S == 1 || mlo == mhi) {
+        if (mhi == kNoDensity) full_tile<SOFT, kNoDensity, kNoDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        else if (mhi == kDensity) full_tile<SOFT, kDensity, kDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kExact, kExact, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+    } else if (mlo == kNoDensity) {
+        if (mhi == kDensity) full_tile<SOFT, kNoDensity, kDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kNoDensity, kExact, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+    } else if (mlo == kDensity) {
+        if (mhi == kNoDensity) full_tile<SOFT, kDensity, kNoDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kDensity, kExact, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+    } else {
+        if (mhi == kNoDensity) full_tile<SOFT, kExact, kNoDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kExact, kDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
     }
 }
 
@@ -665,7 +691,11 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
     const int nsink = w.nsink;  // 64*S, or fewer for a masked tail tile
     const SinkCells sc = sink_cells(v, s);
     float kx[S], ky[S], kz[S];
-    float blx = __builtin_huge_valf(), bly = blx, blz = blx, bhx = -blx, bhy = -blx, bhz = -blx;
+    // The sinks' box: x and y over all slots, z per half of the slots (the
+    // slots run along the pencil, z), so each half of a tile can take its
+    // own pair form.
+    float blx = __builtin_huge_valf(), bly = blx, bhx = -blx, bhy = -blx;
+    float blz0 = blx, bhz0 = -blx, blz1 = blx, bhz1 = -blx;
 #pragma unroll
     for (int u = 0; u < S; u++) {
         // Lanes past nsink repeat sink 0 (inside the box, results not stored).
@@ -676,17 +706,26 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
         kz[u] = k.z;
         blx = fminf(blx, k.x);
         bly = fminf(bly, k.y);
-        blz = fminf(blz, k.z);
         bhx = fmaxf(bhx, k.x);
         bhy = fmaxf(bhy, k.y);
-        bhz = fmaxf(bhz, k.z);
+        if (u < S / 2) {
+            blz0 = fminf(blz0, k.z);
+            bhz0 = fmaxf(bhz0, k.z);
+        } else {
+            blz1 = fminf(blz1, k.z);
+            bhz1 = fmaxf(bhz1, k.z);
+        }
     }
     blx = wave_min(blx);
     bly = wave_min(bly);
-    blz = wave_min(blz);
     bhx = wave_max(bhx);
     bhy = wave_max(bhy);
-    bhz = wave_max(bhz);
+    blz1 = wave_min(blz1);
+    bhz1 = wave_max(bhz1);
+    if (S > 1) {
+        blz0 = wave_min(blz0);
+        bhz0 = wave_max(bhz0);
+    }
 
     float hx[S], hy[S], hz[S], lx[S], ly[S], lz[S], hw[S];
 #pragma unroll
@@ -709,16 +748,21 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
             const NF_CONST float4 *sp = src + st + b * kWave;
             const int cnt8 = (min(kWave, n - b * kWave) + 7) & ~7;
             const int64_t gb = (int64_t)st / kWave + b;
-            const int mode = tile_mode<SOFT>(cx, box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]));
+            const float4 ql = box[2 * gb], qh = box[2 * gb + 1];
+            const float gx = fmaxf(0.f, fmaxf(ql.x - bhx, blx - qh.x));
+            const float gy = fmaxf(0.f, fmaxf(ql.y - shy, sly - qh.y));
+            const float gxy = gx * gx + gy * gy;
+            const float gz1 = fmaxf(0.f, fmaxf(ql.z - bhz1, blz1 - qh.z));
+            const int mhi = tile_mode<SOFT>(cx, gxy + gz1 * gz1);
+            int mlo = mhi;
+            if (S > 1) {
+                const float gz0 = fmaxf(0.f, fmaxf(ql.z - bhz0, blz0 - qh.z));
+                mlo = tile_mode<SOFT>(cx, gxy + gz0 * gz0);
+            }
             float tx[S], ty[S], tz[S], tw[S];
 #pragma unroll
             for (int u = 0; u < S; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
-            if (mode == kNoDensity)
-                full_tile<SOFT, kNoDensity, S>(sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
-            else if (mode == kDensity)
-                full_tile<SOFT, kDensity, S>(sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
-            else
-                full_tile<SOFT, kExact, S>(sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
+            mixed_tile<SOFT, S>(mlo, mhi, sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
 #pragma unroll
             for (int u = 0; u < S; u++) {
                 merge(hx[u], lx[u], tx[u]);
