@@ -744,21 +744,34 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
         for (int u = 0; u < S; u++) kyy[u] = (float)((double)ky[u] - yo);
         const float sly = (float)((double)bly - yo), shy = (float)((double)bhy - yo);
         const int nblk = (n + kWave - 1) / kWave;
+        // Lane b decides the pair forms of block b of this source pencil (64
+        // blocks per pass): one vector pass instead of a wave-uniform box test
+        // and a dependent box load per block.
+        int modes = 0;  // lane i: mlo | mhi << 2 of block b + i, for the pass starting at block b
         for (int b = 0; b < nblk; b++) {
+            if ((b & (kWave - 1)) == 0) {
+                const int bl = b + lane;
+                modes = 0;
+                if (bl < nblk) {
+                    const int64_t gb = (int64_t)st / kWave + bl;
+                    const float4 ql = box[2 * gb], qh = box[2 * gb + 1];
+                    const float gx = fmaxf(0.f, fmaxf(ql.x - bhx, blx - qh.x));
+                    const float gy = fmaxf(0.f, fmaxf(ql.y - shy, sly - qh.y));
+                    const float gxy = gx * gx + gy * gy;
+                    const float gz1 = fmaxf(0.f, fmaxf(ql.z - bhz1, blz1 - qh.z));
+                    const int mh = tile_mode<SOFT>(cx, gxy + gz1 * gz1);
+                    int ml = mh;
+                    if (S > 1) {
+                        const float gz0 = fmaxf(0.f, fmaxf(ql.z - bhz0, blz0 - qh.z));
+                        ml = tile_mode<SOFT>(cx, gxy + gz0 * gz0);
+                    }
+                    modes = ml | (mh << 2);
+                }
+            }
+            const int pm = __builtin_amdgcn_readlane(modes, b & (kWave - 1));
+            const int mlo = pm & 3, mhi = pm >> 2;
             const NF_CONST float4 *sp = src + st + b * kWave;
             const int cnt8 = (min(kWave, n - b * kWave) + 7) & ~7;
-            const int64_t gb = (int64_t)st / kWave + b;
-            const float4 ql = box[2 * gb], qh = box[2 * gb + 1];
-            const float gx = fmaxf(0.f, fmaxf(ql.x - bhx, blx - qh.x));
-            const float gy = fmaxf(0.f, fmaxf(ql.y - shy, sly - qh.y));
-            const float gxy = gx * gx + gy * gy;
-            const float gz1 = fmaxf(0.f, fmaxf(ql.z - bhz1, blz1 - qh.z));
-            const int mhi = tile_mode<SOFT>(cx, gxy + gz1 * gz1);
-            int mlo = mhi;
-            if (S > 1) {
-                const float gz0 = fmaxf(0.f, fmaxf(ql.z - bhz0, blz0 - qh.z));
-                mlo = tile_mode<SOFT>(cx, gxy + gz0 * gz0);
-            }
             float tx[S], ty[S], tz[S], tw[S];
 #pragma unroll
             for (int u = 0; u < S; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
