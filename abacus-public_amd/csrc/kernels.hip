@@ -590,9 +590,8 @@ __device__ __forceinline__ SinkCells sink_cells(const SicView &v, int s) {
     }
     return sc;
 }
-// Sink i of the pencil exactly as k_gather_sinks<true> puts it in a sink
-// stream: its slab position with z moved into the pencil's central cell by
-// the FP64 offset (c - R)/cpd.
+// Sink i of the pencil: its slab position with z moved into the pencil's
+// central cell by the FP64 offset (c - R)/cpd, rounded once.
 __device__ __forceinline__ float3 pencil_sink(const SicView &v, const SinkCells &sc, int i, double invcpd) {
     // The cell holding sink i, by selects over the (wave-uniform) cell table:
     // no indexed private array.
