@@ -74,16 +74,18 @@ __device__ __forceinline__ uint64_t sink_pencil_sources(const GPlanIn &in, int j
     return n;
 }
 
-// The work items of a pencil of n sinks for register tile S, as the runtime's
-// host build_items makes them: `full` tiles of 64 S sinks (kind 0), then
-// one-wave tiles (kind 2) while at least kTailLanes sinks remain -- q of 64
-// sinks and, if the rest s is at least kTailLanes, one of s -- then one short
-// tail of s sinks (kind 1) if 0 < s < kTailLanes.
+// The work items of a pencil of n sinks (meeting nsrc sources) for register
+// tile S, as the runtime's host build_items makes them: `full` tiles of 64 S
+// sinks (kind 0), then one-wave tiles (kind 2) while at least kTailLanes
+// sinks remain -- q of 64 sinks and, if the rest s is at least kTailLanes,
+// one of s -- then one short tail of s sinks (kind 1) if 0 < s < kTailLanes.
+// A pencil whose register tile would cost more than split_cost takes no
+// register tiles: one-wave tiles from its first sink.
 struct ItemSplit {
     int full, tile, n2, q, s, tail;
-    __device__ ItemSplit(int n, int S) {
+    __device__ ItemSplit(int n, int S, uint64_t nsrc, uint64_t split_cost) {
         tile = kB * S;
-        full = n / tile;
+        full = (S > 1 && (uint64_t)tile * nsrc > split_cost) ? 0 : n / tile;
         const int r = n - full * tile;
         q = r / kB;
         s = r - q * kB;
@@ -160,12 +162,10 @@ __global__ __launch_bounds__(kT) void k_plan_a(GPlanIn in, GPlanTotals *tot) {
         sink = (uint64_t)n;
         sblk = (uint64_t)nblocks(n);
         ch = (uint64_t)nblocks(row[k].count);  // pencil k starts with cell z = k: its chunks are that cell's
-        it[0] = (uint64_t)ItemSplit(n, 1).count();
-        it[1] = (uint64_t)ItemSplit(n, 2).count();
-        it[2] = (uint64_t)ItemSplit(n, 4).count();
+        const uint64_t nsrc = sink_pencil_sources(in, j, k);
+        for (int ti = 0; ti < 3; ti++) it[ti] = (uint64_t)ItemSplit(n, 1 << ti, nsrc, in.split_cost[ti]).count();
         const int bin = n < 64 ? 0 : n < 256 ? 1 : n < 1024 ? 2 : 3;
         for (int i = 0; i < 4; i++) occ[i] = bin == i ? (uint64_t)n : 0;
-        const uint64_t nsrc = sink_pencil_sources(in, j, k);
         d = (uint64_t)n * nsrc;
         pd = sblk * kB * nsrc;
     } else if (t < P + Q) {
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) {
                 const uint64_t nsrc = sink_pencil_sources(in, j, k);
                 for (int ti = 0; ti < 3; ti++) {
                     if (!out.items[ti]) continue;
-                    const ItemSplit sp(n, 1 << ti);
+                    const ItemSplit sp(n, 1 << ti, nsrc, in.split_cost[ti]);
                     int32_t *h = lh + ti * kPlanBuckets;
                     if (sp.full) atomicAdd(h + cost_bucket((uint64_t)sp.tile * nsrc), sp.full);
                     if (sp.n2) atomicAdd(h + cost_bucket((uint64_t)kB * nsrc), sp.n2);
@@ -299,7 +299,7 @@ __device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_
     const uint64_t nsrc = n > 0 ? sink_pencil_sources(in, s / in.cpd, s % in.cpd) : 0;
     int bk[3][3], lo[3][3];  // per tile: bucket and local offset of the full, one-wave and tail runs
     for (int ti = 0; ti < 3; ti++) {
-        const ItemSplit sp(n, 1 << ti);
+        const ItemSplit sp(n, 1 << ti, nsrc, in.split_cost[ti]);
         const bool on = out.items[ti] != nullptr && n > 0;
         bk[ti][0] = cost_bucket((uint64_t)sp.tile * nsrc);
         bk[ti][1] = cost_bucket((uint64_t)kB * nsrc);
@@ -318,7 +318,7 @@ __device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_
     for (int ti = 0; ti < 3; ti++) {
         WorkItem *items = out.items[ti];
         if (!items) continue;
-        const ItemSplit sp(n, 1 << ti);
+        const ItemSplit sp(n, 1 << ti, nsrc, in.split_cost[ti]);
         const int32_t *base = lc + ti * kPlanBuckets;
         if (sp.full) {
             const int p = boff[ti][bk[ti][0]] + base[bk[ti][0]] + lo[ti][0];

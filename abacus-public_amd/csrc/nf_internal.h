@@ -135,6 +135,7 @@ struct GPlanIn {
     int32_t cpd, R, W, jw, j_low;
     const nf_cellinfo *ci[NF_MAX_WIDTH];
     uint64_t nslab[NF_MAX_WIDTH];  // particles per slab: every planned cell must lie inside its slab
+    uint64_t split_cost[3];        // per S = 1, 2, 4: register tiles above this cost become one-wave tiles
 };
 // Totals of phase A (device; atomically accumulated, so zeroed first).
 struct GPlanTotals {

@@ -428,8 +428,23 @@ uint16_t cost_bucket(uint64_t cost) {
 // The batched work list of one SIC, appended in sink-pencil order z then y
 // (pencils adjacent in y share 2R of their 2R+1 source pencils), with each
 // item's cost bucket: per sink pencil, full register tiles of 64*S sinks,
-// then the tail.
-void build_items(const nf_sic &s, int S, int sic, std::vector<WorkItem> &items, std::vector<uint16_t> &bucket) {
+// then the tail.  A pencil whose register tile would cost more than
+// max_tile_cost (sinks x sources) takes one-wave tiles of 64 sinks instead:
+// in a launch with little work per wave, one heavy tile would otherwise
+// outlast every other wave.  Each sink's arithmetic is the same in both.
+constexpr uint64_t kSplitMinCost = 1u << 18;  // a tile below this ends in well under a millisecond
+
+// The cost above which a register tile of S sinks per lane is split, for a
+// launch of `interactions` in all: the work per wave of its persistent grid
+// (CUs x 4 SIMDs x the kernel's waves per SIMD), at least kSplitMinCost.
+uint64_t tile_split_cost(int softening, int S, uint64_t interactions) {
+    if (S <= 1) return UINT64_MAX;
+    const uint64_t waves = (uint64_t)std::max(1, device_cu_count()) * 4 *
+                           (uint64_t)std::max(1, batched_kernel_waves(softening, S));
+    return std::max<uint64_t>(kSplitMinCost, interactions / waves);
+}
+void build_items(const nf_sic &s, int S, int sic, uint64_t max_tile_cost, std::vector<WorkItem> &items,
+                 std::vector<uint16_t> &bucket) {
     const int tile = NF_BLOCK_SIZE * S;
     const int W = s.nfwidth;
     for (int z = 0; z < s.Nk; z++)
@@ -440,7 +455,13 @@ void build_items(const nf_sic &s, int S, int sic, std::vector<WorkItem> &items, 
             uint64_t nsrc = 0;
             for (int y = 0; y < W; y++) nsrc += (uint64_t)s.SourceSetCount[s.SinkSourceInteractionList[W * k + y]];
             int f = 0;
-            if (n >= tile) {
+            if (S > 1 && (uint64_t)tile * nsrc > max_tile_cost) {
+                const uint16_t b = cost_bucket((uint64_t)NF_BLOCK_SIZE * nsrc);
+                for (; f + NF_BLOCK_SIZE <= n; f += NF_BLOCK_SIZE) {
+                    items.push_back(WorkItem{k, f, NF_BLOCK_SIZE, 2 + 4 * sic});
+                    bucket.push_back(b);
+                }
+            } else if (n >= tile) {
                 const uint16_t b = cost_bucket((uint64_t)tile * nsrc);
                 for (; f + tile <= n; f += tile) {
                     items.push_back(WorkItem{k, f, tile, 0 + 4 * sic});
@@ -791,10 +812,16 @@ int plan_phase_a(Runtime &rt, Worker &w, std::vector<Task *> &g, std::vector<Run
     if (rc || (rc = w.pin_tot.ensure(256 * n))) return rc;
     char *sc = static_cast<char *>(w.gscratch.p);
     in.assign(n, GPlanIn{});
+    // Heavy register tiles are split against the group's estimated work (the
+    // three planning phases all use this one threshold, so the item counts of
+    // phase A are those the later phases write).
+    uint64_t group_est = 0;
+    for (Task *t : g) group_est += dp_interactions_estimate(t->sic);
     for (size_t i = 0; i < n; i++) {
         Task &t = *g[i];
         const nf_sic &s = t.sic;
         GPlanIn &gi = in[i];
+        for (int ti = 0; ti < 3; ti++) gi.split_cost[ti] = tile_split_cost(s.softening, 1 << ti, group_est);
         gi.cpd = s.cpd;
         gi.R = s.nfradius;
         gi.W = s.nfwidth;
@@ -878,10 +905,13 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     // distinct sinks-per-lane value to be launched (several while tuning).
     // Items within 1/8 octave of cost count as equal and are kept in sink
     // pencil order (SIC, z, y): a stable counting sort by bucket.
+    uint64_t group_total = 0;
+    for (int i = 0; i < n; i++) group_total += g[i]->sic.DirectTotal;
     auto make_items = [&](int S) {
         std::vector<WorkItem> raw, out;
         std::vector<uint16_t> bucket;
-        for (int i = 0; i < n; i++) build_items(g[i]->sic, S, i, raw, bucket);
+        const uint64_t max_tile_cost = tile_split_cost(s0.softening, S, group_total);
+        for (int i = 0; i < n; i++) build_items(g[i]->sic, S, i, max_tile_cost, raw, bucket);
         std::vector<size_t> start(kItemBuckets + 1, 0);
         for (uint16_t b : bucket) start[b + 1]++;
         for (int b = 0; b < kItemBuckets; b++) start[b + 1] += start[b];
@@ -1727,6 +1757,7 @@ extern "C" int nf_plan_sic_device(int32_t cpd, int32_t R, int32_t slab, int32_t 
     NF_HIP(hipMalloc(&b.cells, cbytes * W));
     NF_HIP(hipMalloc(&b.scr, 256 + sizeof(int32_t) * gpu_plan_part_ints(s.NSinkSets, s.NSourceSets)));
     GPlanIn in{};
+    for (uint64_t &c : in.split_cost) c = UINT64_MAX;  // (no work lists here)
     in.cpd = cpd;
     in.R = R;
     in.W = W;
