@@ -1233,8 +1233,9 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
                                   (int32_t)nitems[k], counter, stream);
         };
         if (trial) {
-            // Time each candidate; the chosen shape runs last so its partials
-            // (identical up to summation order) are the ones co-added.
+            // Time each candidate; the chosen shape runs last (every candidate
+            // writes bitwise the same partials: a sink's sum does not depend on
+            // the sinks per lane).
             tev.resize(kNumCandidates + 1);
             for (auto &ev : tev) NF_HIP(hipEventCreate(&ev));
             NF_HIP(hipEventRecord(tev[0], stream));

@@ -111,6 +111,24 @@ def test_batched_register_tiles_agree(box9, ref9, S):
     assert st["rms"] < 1e-6 and st["max"] < 1e-5, (S, st)
 
 
+def test_register_tiles_bitwise_equal(box9):
+    """A sink's sum does not depend on how many sinks a lane holds (S), on
+    which half of a tile it sits in (each half picks its own pair form), or
+    on a heavy register tile being split into one-wave tiles: S = 1, 2 and 4
+    give bitwise the same accelerations.  The heavy cell (1,500 particles)
+    makes register tiles costly enough to be split."""
+    cpd = 7
+    rng = np.random.default_rng(3)
+    pts = rng.random((cpd ** 3 * 12, 3)) - 0.5
+    heavy = synth.box_from_global(np.concatenate([pts, rng.normal(0.0, 0.01, (1500, 3))]), cpd)
+    for box, slabs in ((box9, [0, 4]), (heavy, [2, 3, 4])):
+        res = {S: run_box(box, "batched", slabs=slabs, sinks_per_lane=S) for S in (1, 2, 4)}
+        for x in slabs:
+            assert np.isfinite(res[1][x]).all()
+            for S in (2, 4):
+                assert np.array_equal(res[1][x], res[S][x]), (box.cpd, x, S)
+
+
 def test_split_sics_equal_whole_slab(box9):
     """A slab planned as 3 SICs over y, or split as ExecuteSlabGPU splits it
     (nf_split_slab, here into several SICs for 3 workers), gives bitwise the
