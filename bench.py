@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--bpd", type=int, default=4, help="worker streams (DirectBPD)")
     ap.add_argument("--sinks-per-lane", type=int, default=0, help="0: the autotuner picks per occupancy histogram")
     ap.add_argument("--max-group", type=int, default=8, help="SICs per persistent launch (batched path)")
+    ap.add_argument("--group-interactions", type=float, default=0,
+                    help="pair interactions after which a launch group takes no more SICs (0: the library's default)")
     ap.add_argument("--gpu-streams", type=int, default=1,
                     help="1: workers share one HIP stream (kernels back to back), 0: one per worker")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
@@ -127,6 +129,7 @@ def main():
         with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path=path, buffers_per_device=args.bpd,
                                  max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
                                  gpu_streams=args.gpu_streams, max_group=args.max_group,
+                                 group_interactions=int(args.group_interactions),
                                  use_graphs=args.graphs or 0) as drv:
             tp = time.time()
             sics = []
@@ -422,7 +425,8 @@ def run_stream(args):
             keep.append(far)
     with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path="batched", buffers_per_device=args.bpd,
                              max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
-                             gpu_streams=args.gpu_streams, max_group=args.max_group, use_graphs=graphs) as drv:
+                             gpu_streams=args.gpu_streams, max_group=args.max_group, use_graphs=graphs,
+                             group_interactions=int(args.group_interactions)) as drv:
         tp = time.time()
         sics = [drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST,
                          pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data for c in range(2 * R + 1)],
