@@ -12,7 +12,7 @@
 //   k_coadd_plain  FP32 co-add of the W partials per sink in the reference's
 //       k order (PencilPlan.cc:140-207 as driven by PencilTask.cu:237-247).
 //   k_batched    persistent kernel draining a work queue sorted by cost;
-//       TwoSum-compensated FP32 per-pencil partials stored as FP64.
+//       FP32 tile sums accumulated into FP64 per-pencil partials.
 //   k_coadd_f64  FP64 reduction of the W partials per sink.
 //
 // Pair kernel, all forms: d = source - sink, a -= g(r) d, w += max(0, b2 - r^2).
@@ -726,9 +726,16 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
         bhz0 = wave_max(bhz0);
     }
 
-    float hx[S], hy[S], hz[S], lx[S], ly[S], lz[S], hw[S];
+    // Each block's FP32 tile sum joins an FP64 partial: a convert and an add
+    // per component, which gfx950 issues at the FP32 rate (a TwoSum merge
+    // into an FP32 (hi, lo) pair is 7 instructions and carries fewer bits).
+    double hx[S], hy[S], hz[S];
+    float hw[S];
 #pragma unroll
-    for (int u = 0; u < S; u++) hx[u] = hy[u] = hz[u] = lx[u] = ly[u] = lz[u] = hw[u] = 0.f;
+    for (int u = 0; u < S; u++) {
+        hx[u] = hy[u] = hz[u] = 0.0;
+        hw[u] = 0.f;
+    }
 
     const int W = v.W;
     for (int y = 0; y < W; y++) {
@@ -777,9 +784,9 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
             mixed_tile<SOFT, S>(mlo, mhi, sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
 #pragma unroll
             for (int u = 0; u < S; u++) {
-                merge(hx[u], lx[u], tx[u]);
-                merge(hy[u], ly[u], ty[u]);
-                merge(hz[u], lz[u], tz[u]);
+                hx[u] += (double)tx[u];
+                hy[u] += (double)ty[u];
+                hz[u] += (double)tz[u];
                 hw[u] += tw[u];
             }
         }
@@ -788,9 +795,9 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
     for (int u = 0; u < S; u++) {
         if (lane + kWave * u >= nsink) continue;
         NF_GLOBAL double *o = out + 4 * (int64_t)(base + lane + kWave * u);
-        o[0] = (double)hx[u] + (double)lx[u];
-        o[1] = (double)hy[u] + (double)ly[u];
-        o[2] = (double)hz[u] + (double)lz[u];
+        o[0] = hx[u];
+        o[1] = hy[u];
+        o[2] = hz[u];
         o[3] = (double)hw[u];
     }
 }
