@@ -52,8 +52,9 @@ typedef enum nf_softening {
 /* Execution path.  PLAIN: one launch per SIC over 64-sink blocks with FP32
  * tile accumulation and an FP32 k-ordered co-add (the reference's structure,
  * PencilKernel.cu:97-189 + PencilPlan.cc:140-207).  BATCHED: one persistent
- * launch per SIC draining a work queue sorted by interaction count, TwoSum
- * compensated FP32 accumulation, FP64 co-add of the per-pencil partials. */
+ * launch per group of SICs draining a work queue sorted by interaction
+ * count, FP32 tile sums accumulated in FP64, FP64 co-add of the per-pencil
+ * partials. */
 typedef enum nf_path { NF_PATH_PLAIN = 0, NF_PATH_BATCHED = 1 } nf_path;
 
 /* Where the slab pointers of an nf_sic live. */
