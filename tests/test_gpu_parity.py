@@ -221,6 +221,29 @@ def test_batched_source_order_regimes():
             assert st["rms"] < lim_rms and st["max"] < lim_max, (p, x, st)
 
 
+def test_radius_one_vs_fp64():
+    """NearFieldRadius 1 (pencils of W = 3 cells): both paths against the FP64
+    oracle summed over the 3x3x3 neighbourhood, at the bars of radius 2."""
+    box = small_uniform(cpd=7, ppc=20, seed=5)
+    eps, b2, _ = params(box)
+    slabs = [0, 3, 6]
+    for path in ("plain", "batched"):
+        with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, nfradius=1, path=path,
+                                 buffers_per_device=2, max_buffer_bytes=1 << 30) as drv:
+            accs = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in slabs}
+            for x in slabs:
+                drv.execute_slab(box, x, accs[x], nsplit=1)
+            for x in slabs:
+                drv.finalize(x)
+        for x in slabs:
+            sl, ix, ce = po.slab_sinks(box, x)
+            ref, soft = po.direct_f64(box, 1, "single_spline", eps, b2, sl, ix, ce, with_soft=True)
+            assert np.isfinite(accs[x]).all(), (path, x)
+            st = summarize(rel_err(accs[x], ref), not_soft_dominated(ref, soft))
+            lim_rms, lim_max = (3e-6, 1e-4) if path == "plain" else (1e-6, 1e-5)
+            assert st["rms"] < lim_rms and st["max"] < lim_max, (path, x, st)
+
+
 def test_minimum_cpd_wraps():
     """cpd = 2R+1: every pencil wraps, every source slab is distinct."""
     box = synth.uniform_box(5 ** 3 * 12, 5, seed=9)
