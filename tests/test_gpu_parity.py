@@ -244,6 +244,28 @@ def test_radius_one_vs_fp64():
             assert st["rms"] < lim_rms and st["max"] < lim_max, (path, x, st)
 
 
+def test_density_disabled():
+    """b2 = 0 turns the FOF density off: w is exactly 0 on both paths and the
+    accelerations meet the FP64 oracle (computed with b2 = 0)."""
+    box = small_uniform(cpd=7, ppc=20, seed=8)
+    eps, _, _ = params(box)
+    slabs = [1, 4]
+    for path in ("plain", "batched"):
+        with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, 0.0, nfradius=R, path=path,
+                                 buffers_per_device=2, max_buffer_bytes=1 << 30) as drv:
+            accs = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in slabs}
+            for x in slabs:
+                drv.execute_slab(box, x, accs[x], nsplit=1)
+            for x in slabs:
+                drv.finalize(x)
+        for x in slabs:
+            ref, soft = oracle_slab(box, x, "single_spline", eps, 0.0, with_soft=True)
+            assert np.all(accs[x][:, 3] == 0.0), (path, x)
+            st = summarize(rel_err(accs[x], ref), not_soft_dominated(ref, soft))
+            lim_rms, lim_max = (3e-6, 1e-4) if path == "plain" else (1e-6, 1e-5)
+            assert st["rms"] < lim_rms and st["max"] < lim_max, (path, x, st)
+
+
 def test_minimum_cpd_wraps():
     """cpd = 2R+1: every pencil wraps, every source slab is distinct."""
     box = synth.uniform_box(5 ** 3 * 12, 5, seed=9)
