@@ -244,6 +244,32 @@ def test_radius_one_vs_fp64():
             assert st["rms"] < lim_rms and st["max"] < lim_max, (path, x, st)
 
 
+def test_clustered_box_vs_fp64():
+    """A clustered box (Plummer halos in a GRF lattice; cells from empty to
+    several hundred particles): softened pairs are common and pencils mix
+    every source-order regime.  Against the FP64 oracle, with the bars of the
+    precision study: rms, the max over sinks of condition number <= 100, and
+    the max of error / condition number over all sinks (the condition number
+    of a sink is the sum of its pair terms' magnitudes over |a|)."""
+    box = synth.clustered_box(40, 11, seed=2, n_halos=60)
+    assert box.counts().max() > 128
+    eps, b2, _ = params(box)
+    slabs = [0, 5, 10]
+    res = {p: run_box(box, p, slabs=slabs) for p in ("plain", "batched")}
+    for x in slabs:
+        sl, ix, ce = po.slab_sinks(box, x)
+        ref, soft, mag = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce, with_soft=True, with_abs=True)
+        keep = not_soft_dominated(ref, soft)
+        cond = mag / np.maximum(np.linalg.norm(ref[:, :3], axis=1), 1e-300)
+        for p, (lim_rms, lim_well, lim_cond) in (("plain", (3e-6, 1e-4, 3e-6)), ("batched", (1e-6, 1e-5, 1e-6))):
+            assert np.isfinite(res[p][x]).all(), p
+            rel = rel_err(res[p][x], ref)
+            st = summarize(rel, keep)
+            well = summarize(rel, keep & (cond <= 100.0))
+            assert st["rms"] < lim_rms and well["max"] < lim_well, (p, x, st, well)
+            assert float((rel[keep] / cond[keep]).max()) < lim_cond, (p, x)
+
+
 def test_density_disabled():
     """b2 = 0 turns the FOF density off: w is exactly 0 on both paths and the
     accelerations meet the FP64 oracle (computed with b2 = 0)."""
