@@ -618,19 +618,22 @@ __device__ __forceinline__ int tile_mode(const BatchCtx &cx, float gap2) {
     return gap2 <= cx.dens2 ? kDensity : kNoDensity;
 }
 
-// One source tile (cnt8 <= 64 sources, a multiple of 8) against the S sinks
+// One source tile (cnt <= 64 sources) against the S sinks
 // of every lane; sources are wave-uniform loads (scalar cache).  The force
 // terms of each batch of 8 sources are summed on their own before they join
 // the tile sum: a sum of 8 batch sums of 8 terms carries about 1.6x less
 // FP32 rounding than one running sum of 64 terms, for one add per component
 // per batch.  Sink slots u < S/2 take pair form MLO, the others MHI (S = 1:
 // MHI); every form gives the same value for the pairs its box test admits,
-// so the split changes which code runs, not the result.
+// so the split changes which code runs, not the result.  The last batch
+// takes only the cnt % 8 real sources: the padding sources after them give
+// exact zeros, so leaving them out changes no sum.
 template <int SOFT, int MLO, int MHI, int S>
-__device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, const float *kx, const float *ky,
+__device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt, const float *kx, const float *ky,
                                           const float *kz, const PairConst &pc, float *tx, float *ty, float *tz,
                                           float *tw) {
-    for (int i = 0; i < cnt8; i += 8) {
+    const int full8 = cnt & ~7;
+    for (int i = 0; i < full8; i += 8) {
         float bx[S], by[S], bz[S];
 #pragma unroll
         for (int u = 0; u < S; u++) bx[u] = by[u] = bz[u] = 0.f;
@@ -651,26 +654,46 @@ __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt8, c
             tz[u] += bz[u];
         }
     }
+    if (cnt > full8) {
+        float bx[S], by[S], bz[S];
+#pragma unroll
+        for (int u = 0; u < S; u++) bx[u] = by[u] = bz[u] = 0.f;
+        for (int q = full8; q < cnt; q++) {
+            const float4 sv = sp[q];
+#pragma unroll
+            for (int u = 0; u < S / 2; u++)
+                pair_box<SOFT, MLO>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, bx[u], by[u], bz[u], tw[u]);
+#pragma unroll
+            for (int u = S / 2; u < S; u++)
+                pair_box<SOFT, MHI>(sv.x, sv.y, sv.z, kx[u], ky[u], kz[u], pc, bx[u], by[u], bz[u], tw[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < S; u++) {
+            tx[u] += bx[u];
+            ty[u] += by[u];
+            tz[u] += bz[u];
+        }
+    }
 }
 
 // A tile with the modes of its two halves of sink slots.
 template <int SOFT, int S>
-__device__ __forceinline__ void mixed_tile(int mlo, int mhi, const NF_CONST float4 *sp, int cnt8, const float *kx,
+__device__ __forceinline__ void mixed_tile(int mlo, int mhi, const NF_CONST float4 *sp, int cnt, const float *kx,
                                            const float *ky, const float *kz, const PairConst &pc, float *tx,
                                            float *ty, float *tz, float *tw) {
     if (S == 1 || mlo == mhi) {
-        if (mhi == kNoDensity) full_tile<SOFT, kNoDensity, kNoDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
-        else if (mhi == kDensity) full_tile<SOFT, kDensity, kDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
-        else full_tile<SOFT, kExact, kExact, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        if (mhi == kNoDensity) full_tile<SOFT, kNoDensity, kNoDensity, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
+        else if (mhi == kDensity) full_tile<SOFT, kDensity, kDensity, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kExact, kExact, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
     } else if (mlo == kNoDensity) {
-        if (mhi == kDensity) full_tile<SOFT, kNoDensity, kDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
-        else full_tile<SOFT, kNoDensity, kExact, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        if (mhi == kDensity) full_tile<SOFT, kNoDensity, kDensity, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kNoDensity, kExact, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
     } else if (mlo == kDensity) {
-        if (mhi == kNoDensity) full_tile<SOFT, kDensity, kNoDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
-        else full_tile<SOFT, kDensity, kExact, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        if (mhi == kNoDensity) full_tile<SOFT, kDensity, kNoDensity, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kDensity, kExact, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
     } else {
-        if (mhi == kNoDensity) full_tile<SOFT, kExact, kNoDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
-        else full_tile<SOFT, kExact, kDensity, S>(sp, cnt8, kx, ky, kz, pc, tx, ty, tz, tw);
+        if (mhi == kNoDensity) full_tile<SOFT, kExact, kNoDensity, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
+        else full_tile<SOFT, kExact, kDensity, S>(sp, cnt, kx, ky, kz, pc, tx, ty, tz, tw);
     }
 }
 
@@ -777,11 +800,11 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
             const int pm = __builtin_amdgcn_readlane(modes, b & (kWave - 1));
             const int mlo = pm & 3, mhi = pm >> 2;
             const NF_CONST float4 *sp = src + st + b * kWave;
-            const int cnt8 = (min(kWave, n - b * kWave) + 7) & ~7;
+            const int cnt = min(kWave, n - b * kWave);
             float tx[S], ty[S], tz[S], tw[S];
 #pragma unroll
             for (int u = 0; u < S; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
-            mixed_tile<SOFT, S>(mlo, mhi, sp, cnt8, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
+            mixed_tile<SOFT, S>(mlo, mhi, sp, cnt, kx, kyy, kz, cx.pc, tx, ty, tz, tw);
 #pragma unroll
             for (int u = 0; u < S; u++) {
                 hx[u] += (double)tx[u];
