@@ -967,11 +967,12 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
 constexpr int kRemSinks = 2;  // sinks per 16-lane group per pass of a short tail
 
 // Occupancy target WPE (waves per SIMD): more resident waves hide the v_rsq
-// and scalar-load latencies, as long as the registers fit without spills.
-// The most waves at which each (pair form, sinks per lane) compiles with no
-// VGPR spills for gfx950 (6 waves: 80 VGPRs; 5: 96; 4: 128; 2: 256); the
-// single spline and Plummer forms keep 4 waves at S = 4, which measured
-// faster than 5.
+// and scalar-load latencies, as long as the tile loops' registers fit.  The
+// most waves at which each (pair form, sinks per lane) compiled with no VGPR
+// spills for gfx950 (6 waves: 80 VGPRs; 5: 96; 4: 128; 2: 256); the single
+// spline and Plummer forms keep 4 waves at S = 4, which measured faster than
+// 5.  With FP64 tile accumulation the S = 1 forms spill a few words in their
+// item set-up, outside the tile loops, at no measured cost.
 constexpr int batched_waves_per_eu(int soft, int s) {
     switch (soft) {
         case NF_SOFT_SINGLE_SPLINE: return s == 4 ? 4 : 6;
