@@ -156,6 +156,7 @@ struct Worker {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     hipEvent_t plan_ready = nullptr;  // the plan image upload (copy stream) is done
     hipStream_t pstream = nullptr;     // device planning (nf_submit_slab)
+    bool own_streams = true;           // stream and pstream are this worker's (else the runtime's)
     hipEvent_t gplan_ready = nullptr;  // the device-planned arrays and work lists are written
     DevBuf gscratch;                   // the device planner's totals and scratch
     PinBuf pin_tot;                    // phase A's totals, read back
@@ -199,7 +200,8 @@ struct Runtime {
         double active_s = 0.0;
         uint64_t direct = 0, padded = 0, to_dev = 0, from_dev = 0, tasks = 0, groups = 0;
     } tp;
-    hipStream_t shared = nullptr;  // when gpu_streams == 1: the one stream all workers use
+    hipStream_t shared = nullptr;   // when gpu_streams == 1: the one stream all workers use
+    hipStream_t pstream = nullptr;  // when gpu_streams == 1: the workers' one planning stream
     std::mutex enqueue;            // orders whole task sequences on the shared stream
 
     // Host-mode slabs resident in HBM.  A slot is reusable by address only
@@ -1456,12 +1458,21 @@ void worker_main(Runtime *rt, Worker *w) {
     }
 }
 
-int init_worker(Worker &w) {
+// With a shared compute stream (gpu_streams = 1) the workers also share one
+// planning stream, so the runtime holds four streams in all (compute, upload,
+// download, planning): as many as HIP's hardware queues per process, so no
+// stream queues behind another's work.  Otherwise each worker has its own.
+int init_worker(Worker &w, hipStream_t shared_pstream) {
     NF_HIP(hipSetDevice(w.device));
-    NF_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+    w.own_streams = shared_pstream == nullptr;
+    if (w.own_streams) {
+        NF_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+        NF_HIP(hipStreamCreateWithFlags(&w.pstream, hipStreamNonBlocking));
+    } else {
+        w.pstream = shared_pstream;
+    }
     for (auto &e : w.ev) NF_HIP(hipEventCreate(&e));
     NF_HIP(hipEventCreateWithFlags(&w.plan_ready, hipEventDisableTiming));
-    NF_HIP(hipStreamCreateWithFlags(&w.pstream, hipStreamNonBlocking));
     NF_HIP(hipEventCreateWithFlags(&w.gplan_ready, hipEventDisableTiming));
     w.ready = true;
     return NF_OK;
@@ -1486,8 +1497,11 @@ void destroy_worker(Worker &w) {
         if (e) (void)hipEventDestroy(e);
     if (w.plan_ready) (void)hipEventDestroy(w.plan_ready);
     if (w.gplan_ready) (void)hipEventDestroy(w.gplan_ready);
-    if (w.pstream) (void)hipStreamDestroy(w.pstream);
-    if (w.stream) (void)hipStreamDestroy(w.stream);
+    if (w.own_streams) {
+        if (w.pstream) (void)hipStreamDestroy(w.pstream);
+        if (w.stream) (void)hipStreamDestroy(w.stream);
+    }
+    w.stream = w.pstream = nullptr;
     w.ready = false;
 }
 
@@ -1563,37 +1577,39 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
     if (ms < 2 * p->cpd) return fail(NF_ERR_CAPACITY, "max_buffer_bytes too small for one skewer per SIC");
     if (max_sink_blocks) *max_sink_blocks = ms;
     if (max_source_blocks) *max_source_blocks = (int32_t)std::min<double>((double)ms * W, INT32_MAX / NF_BLOCK_SIZE);
+    if (p->gpu_streams != 0 && p->gpu_streams != 1) return fail(NF_ERR_INVALID, "gpu_streams must be 0 or 1");
+    // Streams first (compute, upload, download, planning), then the workers.
+    auto destroy_streams = [&] {
+        for (hipStream_t st : {rt->shared, rt->pstream, rt->cache.up, rt->cache.down})
+            if (st) (void)hipStreamDestroy(st);
+        rt->shared = rt->pstream = rt->cache.up = rt->cache.down = nullptr;
+    };
+    {
+        hipError_t e = hipSuccess;
+        if (p->gpu_streams == 1) e = hipStreamCreateWithFlags(&rt->shared, hipStreamNonBlocking);
+        auto &C = rt->cache;
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&C.up, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&C.down, hipStreamNonBlocking);
+        if (e == hipSuccess && p->gpu_streams == 1) e = hipStreamCreateWithFlags(&rt->pstream, hipStreamNonBlocking);
+        for (int k = 0; k < Runtime::kStages && e == hipSuccess; k++)
+            e = hipEventCreateWithFlags(&C.stage_done[k], hipEventDisableTiming);
+        if (e != hipSuccess) {
+            destroy_streams();
+            return fail(NF_ERR_HIP, std::string("stream setup: ") + hipGetErrorString(e));
+        }
+    }
     for (int i = 0; i < p->buffers_per_device; i++) {
         auto w = std::make_unique<Worker>();
         w->id = i;
         w->device = 0;
-        int rc = init_worker(*w);
+        int rc = init_worker(*w, rt->pstream);
         if (rc) {
             for (auto &x : rt->workers) destroy_worker(*x);
+            destroy_worker(*w);
+            destroy_streams();
             return rc;
         }
         rt->workers.push_back(std::move(w));
-    }
-    {
-        auto &C = rt->cache;
-        hipError_t e = hipStreamCreateWithFlags(&C.up, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&C.down, hipStreamNonBlocking);
-        for (int k = 0; k < Runtime::kStages && e == hipSuccess; k++)
-            e = hipEventCreateWithFlags(&C.stage_done[k], hipEventDisableTiming);
-        if (e != hipSuccess) {
-            for (auto &x : rt->workers) destroy_worker(*x);
-            return fail(NF_ERR_HIP, std::string("copy stream setup: ") + hipGetErrorString(e));
-        }
-    }
-    if (p->gpu_streams == 1) {
-        hipError_t e = hipStreamCreateWithFlags(&rt->shared, hipStreamNonBlocking);
-        if (e != hipSuccess) {
-            for (auto &x : rt->workers) destroy_worker(*x);
-            return fail(NF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-        }
-    } else if (p->gpu_streams != 0) {
-        for (auto &x : rt->workers) destroy_worker(*x);
-        return fail(NF_ERR_INVALID, "gpu_streams must be 0 or 1");
     }
     for (auto &w : rt->workers) w->th = std::thread(worker_main, rt.get(), w.get());
     g_rt = rt.release();
@@ -1611,10 +1627,11 @@ extern "C" int nf_reset(void) {
     for (auto &w : g_rt->workers)
         if (w->th.joinable()) w->th.join();
     for (auto &w : g_rt->workers) destroy_worker(*w);
-    if (g_rt->shared) {
-        (void)hipStreamSynchronize(g_rt->shared);
-        (void)hipStreamDestroy(g_rt->shared);
-    }
+    for (hipStream_t st : {g_rt->shared, g_rt->pstream})
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
     {
         auto &C = g_rt->cache;
         for (hipStream_t st : {C.up, C.down})
