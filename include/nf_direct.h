@@ -199,9 +199,10 @@ typedef struct nf_setup_params {
     int32_t sinks_per_lane;       /* batched path register tile: 1, 2 or 4 (0: auto) */
     int32_t source_tile;          /* batched path TwoSum tile depth (0: auto) */
     int32_t gpu_streams;          /* HIP streams the workers enqueue on: 0 one per
-                                     worker (copies overlap kernels); 1 one shared
-                                     stream (tasks run back to back on the GPU while
-                                     the workers still prepare them in parallel) */
+                                     worker; 1 one shared stream (tasks run back to
+                                     back on the GPU while the workers still prepare
+                                     them in parallel) and one shared planning
+                                     stream: measured fastest */
     int32_t max_group;            /* batched path: SICs queued together may share one
                                      persistent launch, up to this many (0: 8) */
     int32_t use_graphs;           /* 1: each group's launch sequence (gathers, direct
