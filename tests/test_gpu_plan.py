@@ -209,3 +209,30 @@ def test_submit_slab_rejects_cells_outside_slab():
         ok = np.full((box.slab(2).n, 4), np.nan, np.float32)
         drv.submit_slab(box, 2, acc_ptr=ok.ctypes.data, blocking=True).release()
     assert np.isnan(acc).all() and np.isfinite(ok).all()
+
+
+def test_shared_streams_match_per_worker_streams():
+    """gpu_streams = 1 (one compute stream and one planning stream shared by
+    the workers: the layout the benchmark and the integration shim use) gives
+    bitwise the accelerations of one stream pair per worker, host- and
+    device-planned, with the workers' groups interleaved on the shared
+    streams."""
+    box = small_uniform(cpd=9, ppc=20, seed=11)
+    eps, b2, _ = params(box)
+    out = {}
+    for gs in (0, 1):
+        with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=2,
+                                 max_buffer_bytes=2 << 30, sinks_per_lane=2, gpu_streams=gs) as drv:
+            for planner in ("host", "device"):
+                accs = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in range(box.cpd)}
+                for x in range(box.cpd):
+                    drv.execute_slab(box, x, accs[x], nsplit=2, planner=planner)
+                for x in range(box.cpd):
+                    drv.finalize(x)
+                out[(gs, planner)] = accs
+    ref = out[(0, "host")]
+    for key, accs in out.items():
+        for x in range(box.cpd):
+            assert np.isfinite(accs[x]).all(), key
+            assert np.array_equal(accs[x], ref[x]), (key, x)
+
