@@ -12,6 +12,8 @@
  *   or_kernel_f64       the same forms evaluated in FP64 (no guards)
  *   or_direct_f64       FP64 brute-force direct sum over the (2R+1)^3 cell
  *                       neighbourhood on the exact FP32 cell-relative inputs
+ *   or_floor_f32        the same sum of FP32 pair terms formed as the batched
+ *                       kernel forms them (the error floor of FP32 terms)
  *   or_calc_di          brute-force interaction count (DirectBench.cpp CalcDI)
  *   or_gather_*         restatement of the pencil gathers (PencilPlan.cc)
  *   or_coadd_f32        restatement of the k-ordered set/add co-add
@@ -84,33 +86,40 @@ static inline void or_kernel_gpu_f32(int kind, float sx, float sy, float sz,
     *az -= f * dz;
 }
 
-/* FP64 form of the same kernels, used by the brute-force oracle.  The
- * mathematics follows DirectGPUKernels.cuh without the 1e-32 guard: the
- * single spline is F(u)=u^-3 for u>1 and 10-15u+6u^2 for u<=1 with u=r/eps_int
- * (q=u^2); the self pair (r=0) contributes nothing to a and b2 to w. */
+/* FP64 form of the same kernels, used by the brute-force oracle: the pair
+ * term g(r), with a -= g d.  The mathematics follows DirectGPUKernels.cuh
+ * without the 1e-32 guard: the single spline is F(u)=u^-3 for u>1 and
+ * 10-15u+6u^2 for u<=1 with u=r/eps_int (q=u^2); the self pair (r=0)
+ * contributes nothing to a (g = 0) and b2 to w. */
+static inline double or_g_f64(int kind, double r2, double eps)
+{
+    if (r2 == 0.0) return 0.0;
+    if (kind == OR_SINGLE_SPLINE) {
+        double q = r2 * eps, u = sqrt(q);
+        return (q > 1.0) ? 1.0 / (q * u) : (10.0 - 15.0 * u + 6.0 * q);
+    }
+    if (kind == OR_PLUMMER) {
+        double t = r2 + eps;
+        return 1.0 / (t * sqrt(t));
+    }
+    if (kind == OR_CUBIC_SPLINE) {
+        double r = sqrt(r2), rinv = 1.0 / r, u = eps * r, u3 = u * u * u;
+        if (u >= 2.0) return rinv * rinv * rinv;
+        if (u >= 1.0)
+            return rinv * rinv * rinv * (-1.0 / 15.0 + 8.0 / 3.0 * u3 - 3.0 * u3 * u +
+                                         6.0 / 5.0 * u3 * u * u - 1.0 / 6.0 * u3 * u3);
+        return eps * eps * eps * (4.0 / 3.0 - 6.0 / 5.0 * u * u + 0.5 * u3);
+    }
+    double r = r2 + 1e-16;
+    return 1.0 / (r * sqrt(r) + eps);
+}
+
 static inline void or_kernel_f64(int kind, double dx, double dy, double dz, double eps,
                                  double b2, double *a)
 {
-    double r2 = dx * dx + dy * dy + dz * dz, g;
+    double r2 = dx * dx + dy * dy + dz * dz;
     if (r2 < b2) a[3] += b2 - r2;
-    if (r2 == 0.0) return;
-    if (kind == OR_SINGLE_SPLINE) {
-        double q = r2 * eps, u = sqrt(q);
-        g = (q > 1.0) ? 1.0 / (q * u) : (10.0 - 15.0 * u + 6.0 * q);
-    } else if (kind == OR_PLUMMER) {
-        double s = r2 + eps;
-        g = 1.0 / (s * sqrt(s));
-    } else if (kind == OR_CUBIC_SPLINE) {
-        double r = sqrt(r2), rinv = 1.0 / r, u = eps * r, u3 = u * u * u;
-        if (u >= 2.0) g = rinv * rinv * rinv;
-        else if (u >= 1.0)
-            g = rinv * rinv * rinv * (-1.0 / 15.0 + 8.0 / 3.0 * u3 - 3.0 * u3 * u +
-                                      6.0 / 5.0 * u3 * u * u - 1.0 / 6.0 * u3 * u3);
-        else g = eps * eps * eps * (4.0 / 3.0 - 6.0 / 5.0 * u * u + 0.5 * u3);
-    } else {
-        double r = r2 + 1e-16;
-        g = 1.0 / (r * sqrt(r) + eps);
-    }
+    double g = or_g_f64(kind, r2, eps);
     a[0] -= g * dx;
     a[1] -= g * dy;
     a[2] -= g * dz;
@@ -202,14 +211,18 @@ void or_direct_f64(int cpd, const float *const *pos, const int64_t *nslab,
                         double dy = (double)Q[m + st + p] + oy;
                         double dz = (double)Q[2 * m + st + p] + oz;
                         double r2 = dx * dx + dy * dy + dz * dz;
-                        if (soft_norm && r2 < soft_r2)
-                            or_kernel_f64(kind, dx, dy, dz, eps, 0.0, as);
-                        if (abs_norm) {
-                            double t[4] = {0, 0, 0, 0};
-                            or_kernel_f64(kind, dx, dy, dz, eps, 0.0, t);
-                            mag += sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+                        /* the pair term of or_kernel_f64, evaluated once */
+                        if (r2 < b2) a[3] += b2 - r2;
+                        double g = or_g_f64(kind, r2, eps);
+                        a[0] -= g * dx;
+                        a[1] -= g * dy;
+                        a[2] -= g * dz;
+                        if (soft_norm && r2 < soft_r2) {
+                            as[0] -= g * dx;
+                            as[1] -= g * dy;
+                            as[2] -= g * dz;
                         }
-                        or_kernel_f64(kind, dx, dy, dz, eps, b2, a);
+                        if (abs_norm) mag += fabs(g) * sqrt(r2);
                     }
                 }
             }
@@ -220,6 +233,79 @@ void or_direct_f64(int cpd, const float *const *pos, const int64_t *nslab,
         out[4 * s + 3] = a[3];
         if (soft_norm) soft_norm[s] = sqrt(as[0] * as[0] + as[1] * as[1] + as[2] * as[2]);
         if (abs_norm) abs_norm[s] = mag;
+    }
+}
+
+/* The FP32 pair-term floor of the batched kernel's arithmetic: for each
+ * requested sink, every pair term evaluated in FP32 exactly as the batched
+ * path forms it, the terms summed in FP64.  Compared with or_direct_f64 it
+ * isolates the error that FP32 pair terms alone carry, whatever the
+ * summation: the force sum of a sink can be no more accurate than this.
+ * The frame is the pencil's (extras_PencilOnPencil.cc:80, PencilPlan.cc:35-68,
+ * 226-238, with the batched path's FP64 offsets rounded once): the source x
+ * moved by its slab offset di/cpd, the sink y by -dj/cpd and the sink z by
+ * -dk/cpd, each rounded once to FP32; then d = s - k in FP32,
+ * r2 = fma(dz,dz, fma(dy,dy, fma(dx,dx, tau))), f = 1/sqrt(r2) rounded once
+ * (the hardware v_rsq_f32 is within an ulp of it), g = f*f*f in box units,
+ * or min(f^3, (c1 f + c2) r2 + c3) for softened pairs.  Single spline only;
+ * out is scaled to the reference's units (times e^3) like the co-add. */
+void or_floor_f32(int cpd, const float *const *pos, const int64_t *nslab,
+                  const int64_t *const *start, const int32_t *const *count, int R,
+                  double eps, int64_t nsinks, const int32_t *sink_slab,
+                  const int64_t *sink_index, const int32_t *sink_cell, double *out)
+{
+    const double invcpd = 1.0 / (double)cpd;
+    const double e = eps; /* eps = 1/e_int^2 */
+    const float c1 = (float)(-15.0 * e * e), c2 = (float)(6.0 * e * e * sqrt(e)),
+                c3 = (float)(10.0 * e * sqrt(e));
+    const double out_scale = 1.0 / (e * sqrt(e));
+    const float soft_r2 = (float)(1.0 / e), tau = 1e-24f;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t s = 0; s < nsinks; s++) {
+        int x = sink_slab[s];
+        int64_t i = sink_index[s];
+        int y = sink_cell[s] / cpd, z = sink_cell[s] % cpd;
+        const float *P = pos[x];
+        int64_t n = nslab[x];
+        const float kx = P[i], ky = P[n + i], kz = P[2 * n + i];
+        double a[3] = {0, 0, 0};
+        for (int di = -R; di <= R; di++) {
+            int xs = wrapi(x + di, cpd);
+            const float *Q = pos[xs];
+            int64_t m = nslab[xs];
+            for (int dj = -R; dj <= R; dj++) {
+                int ys = wrapi(y + dj, cpd);
+                const float kyy = (float)((double)ky - dj * invcpd);
+                for (int dk = -R; dk <= R; dk++) {
+                    int zs = wrapi(z + dk, cpd);
+                    const float kzz = (float)((double)kz - dk * invcpd);
+                    int c = ys * cpd + zs;
+                    int64_t st = start[xs][c];
+                    int32_t cn = count[xs][c];
+                    for (int32_t p = 0; p < cn; p++) {
+                        const float sx = (float)((double)Q[st + p] + di * invcpd);
+                        const float dx = sx - kx, dy = Q[m + st + p] - kyy, dz = Q[2 * m + st + p] - kzz;
+                        float r2 = fmaf(dx, dx, tau);
+                        r2 = fmaf(dy, dy, r2);
+                        r2 = fmaf(dz, dz, r2);
+                        const float f = (float)(1.0 / sqrt((double)r2));
+                        float g = f * f;
+                        g = g * f;
+                        if (r2 <= soft_r2) {
+                            const float poly = fmaf(fmaf(c1, f, c2), r2, c3);
+                            g = poly < g ? poly : g;
+                        }
+                        if (dx == 0.0f && dy == 0.0f && dz == 0.0f) continue; /* the self pair: no force */
+                        a[0] -= (double)g * dx;
+                        a[1] -= (double)g * dy;
+                        a[2] -= (double)g * dz;
+                    }
+                }
+            }
+        }
+        out[3 * s + 0] = a[0] * out_scale;
+        out[3 * s + 1] = a[1] * out_scale;
+        out[3 * s + 2] = a[2] * out_scale;
     }
 }
 

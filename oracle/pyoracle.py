@@ -98,6 +98,21 @@ def direct_f64(box, R, kind, eps, b2, sink_slab, sink_index, sink_cell, with_sof
     return tuple([out] + extra) if extra else out
 
 
+def floor_f32(box, R, eps, sink_slab, sink_index, sink_cell):
+    """The FP32 pair-term floor (or_floor_f32, single spline): each sink's
+    force from pair terms formed in FP32 as the batched kernel forms them,
+    summed in FP64 -> (n, 3) float64 in the reference's units."""
+    lib = _load("liboracle.so")
+    a = _BoxArgs(box)
+    ss = np.ascontiguousarray(sink_slab, np.int32)
+    si = np.ascontiguousarray(sink_index, np.int64)
+    sc = np.ascontiguousarray(sink_cell, np.int32)
+    out = np.zeros((ss.size, 3))
+    lib.or_floor_f32(C.c_int(box.cpd), a.pos, _p(a.nslab), a.start, a.count, C.c_int(R),
+                     C.c_double(float(eps)), C.c_int64(ss.size), _p(ss), _p(si), _p(sc), _p(out))
+    return out
+
+
 def slab_sinks(box, x):
     """(slab, index, cell) arrays for every particle of slab x."""
     s = box.slab(x)
@@ -105,11 +120,12 @@ def slab_sinks(box, x):
     return np.full(s.n, x % box.cpd, np.int32), np.arange(s.n, dtype=np.int64), cell
 
 
-def sample_sinks(box, frac, seed):
-    """A seeded random subsample of all sinks of the box."""
+def sample_sinks(box, frac, seed, slabs=None):
+    """A seeded random subsample of the sinks of the box (of the listed slabs
+    only, if given): round(frac * n) of each slab's n, at least one."""
     rng = np.random.default_rng(seed)
     sl, ix, ce = [], [], []
-    for x in range(box.cpd):
+    for x in (range(box.cpd) if slabs is None else [s % box.cpd for s in slabs]):
         s = box.slab(x)
         k = max(1, int(round(frac * s.n))) if s.n else 0
         if k == 0:

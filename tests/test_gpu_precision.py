@@ -1,10 +1,14 @@
 """Precision study on the 64^3, CPD 17 uniform workload (all 262,144 sinks).
 
 Variants: the plain path (FP32 as the reference accumulates it: BLOCK_ACCUM
-tile sums, FP32 offsets, FP32 co-add), the batched path (TwoSum-compensated
-FP32 per pencil, FP64 cell offsets, FP64 co-add), the reference CPU directs
-(restated, oracle/cpu_directs.c), all against the FP64 direct-summation
-oracle on the same FP32 cell-relative inputs.
+tile sums, FP32 offsets, FP32 co-add), the batched path (FP32 sums of
+8-source batches and 64-source tiles, the tile sums accumulated in FP64, FP64
+cell offsets, FP64 co-add; short tails keep TwoSum-compensated FP32 pairs),
+the reference CPU directs (restated, oracle/cpu_directs.c), all against the
+FP64 direct-summation oracle on the same FP32 cell-relative inputs.  Beside
+them the FP32 pair-term floor (oracle or_floor_f32): the batched kernel's
+pair terms formed in FP32 and summed exactly, the best any summation of
+those terms can do.
 """
 import json
 import os
@@ -44,6 +48,8 @@ def test_precision_study_uniform64():
         mags.append(m)
         cpu.append(po.execute_slab_cpu(box, R, "single_spline", eps, x)[0])
     ref, soft, mag, cpu = (np.concatenate(v) for v in (refs, softs, mags, cpu))
+    parts = [po.slab_sinks(box, x) for x in range(box.cpd)]
+    floor = po.floor_f32(box, R, eps, *(np.concatenate([p[i] for p in parts]) for i in range(3)))
     keep = not_soft_dominated(ref, soft)
     # Condition number of each sink's force sum: sum of term magnitudes / |a|.
     # FP32 terms carry relative errors of ~1e-7, so a sink's attainable
@@ -54,15 +60,19 @@ def test_precision_study_uniform64():
               "softening_dominated_sinks": int((~keep).sum()),
               "condition_number": {"median": float(np.median(cond)), "p99": float(np.percentile(cond, 99)),
                                    "max": float(cond.max()), "sinks_above_100": int((cond > 100).sum())}}
-    for name, acc in (("plain", res["plain"]), ("batched", res["batched"]), ("cpu_directs", cpu)):
+    fl = np.concatenate([floor, ref[:, 3:]], axis=1)
+    for name, acc in (("plain", res["plain"]), ("batched", res["batched"]), ("cpu_directs", cpu),
+                      ("fp32_floor", fl)):
         rel = rel_err(acc, ref)
         report[name] = {"all": summarize(rel, tail=True),
                         "outside_softening_dominated": summarize(rel, keep, tail=True),
                         "condition_number_le_100": summarize(rel, well, tail=True),
                         "max_error_over_condition_number": float((rel[keep] / cond[keep]).max())}
-        if name != "cpu_directs":
+        if name in res:
             report[name]["kernel_interactions_per_s"] = rate[name]
-            report[name]["vs_cpu_directs"] = summarize(rel_err(acc, cpu))
+            report[name]["vs_cpu_directs"] = summarize(rel_err(acc, cpu), keep, tail=True)
+            # distance from the exactly summed FP32 pair terms, per sink
+            report[name]["vs_fp32_floor"] = summarize(rel_err(acc, fl), keep, tail=True)
     print(json.dumps(report))
     out = os.environ.get("NF_PRECISION_REPORT")
     if out:
@@ -77,6 +87,12 @@ def test_precision_study_uniform64():
     # better than both the reference-structured FP32 path and the reference
     # CPU directs on the same inputs; on sinks with condition number <= 100
     # the max is within 1e-5.
+    f = report["fp32_floor"]["outside_softening_dominated"]
+    # The floor itself misses max <= 1e-5 on this box (tests/test_oracle.py);
+    # the batched path stays within 1.5x of it, and within 1e-5 of the
+    # exactly summed FP32 terms on every sink that is not softening-dominated.
+    assert f["max"] > 1e-5 and b["max"] <= 1.5 * f["max"], (b, f)
+    assert report["batched"]["vs_fp32_floor"]["max"] <= 1e-5, report["batched"]["vs_fp32_floor"]
     assert b["rms"] <= 4e-7 and b["max"] <= 3e-5, b
     assert b["rms"] < p["rms"] and b["rms"] < c["rms"]
     assert b["max"] < p["max"] and b["max"] < c["max"]
@@ -85,5 +101,7 @@ def test_precision_study_uniform64():
     assert bw["max"] <= 1e-5 and bw["n"] >= 0.99 * b["n"], bw
     assert report["batched"]["max_error_over_condition_number"] <= 1e-6
     assert report["batched"]["vs_cpu_directs"]["rms"] <= 1e-6
+    # the max against the CPU directs is theirs: they err by up to ~4e-5 against FP64 here
+    assert report["batched"]["vs_cpu_directs"]["max"] <= 1.2 * report["cpu_directs"]["outside_softening_dominated"]["max"]
     # the FOF density w against FP64 (self term included)
     np.testing.assert_allclose(res["batched"][:, 3], ref[:, 3], rtol=2e-6, atol=1e-6 * float(b2))

@@ -222,6 +222,72 @@ def test_planted_offset_mistake_is_caught():
             assert not np.allclose(a[:3], -expect, rtol=1e-3, atol=0)
 
 
+# ------------------------------------------------ the FP32 pair-term floor
+
+@pytest.mark.parametrize("p,q", [
+    ([0.01, 0.02, -0.03], [0.06, -0.01, 0.04]),       # neighbouring cells
+    ([-0.49, 0.1, 0.1], [0.47, 0.12, 0.09]),          # across the periodic wrap in x
+    ([0.2, -0.495, 0.3], [0.2, 0.49, 0.28]),          # across the wrap in y
+    ([0.0, 0.0, 0.496], [0.01, 0.0, -0.47]),          # across the wrap in z
+])
+def test_fp32_floor_two_particles(p, q):
+    """One pair term per sink (condition number 1): the FP32 term is the FP64
+    one to a few FP32 roundings, across every periodic wrap and frame offset."""
+    cpd = 9
+    box = two_particle_box(cpd, p, q)
+    eps = 1 / 1e-4 ** 2
+    for x in range(cpd):
+        if box.slab(x).n:
+            sl, ix, ce = po.slab_sinks(box, x)
+            ref = po.direct_f64(box, R, "single_spline", eps, 0.0, sl, ix, ce)
+            fl = po.floor_f32(box, R, eps, sl, ix, ce)
+            np.testing.assert_allclose(fl, ref[:, :3], rtol=2e-6)
+
+
+def test_fp32_floor_is_bounded_by_the_condition_number():
+    """On a random box every sink's floor error is below 1e-6 x its condition
+    number (about 16 FP32 roundings per pair term): the emulated terms carry
+    FP32-level errors only.  A frame mistake (a dropped or mis-signed cell
+    offset) puts O(1) errors on every sink and fails this; so does a planted
+    one here, the y offset's sign flipped by mirroring a slab's y."""
+    box = small_uniform(cpd=7, ppc=25, seed=4)
+    eps, b2, _ = params(box)
+    sl, ix, ce = po.slab_sinks(box, 3)
+    ref, mag = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce, with_abs=True)
+    cond = mag / np.linalg.norm(ref[:, :3], axis=1)
+    rel = rel_err(po.floor_f32(box, R, eps, sl, ix, ce), ref)
+    assert np.all(rel <= 1e-6 * cond), (rel / cond).max()
+    assert summarize(rel)["rms"] < 5e-7
+    s = box.slab(4)
+    bad = synth.Box(box.cpd, [synth.Slab(np.concatenate([s.xyz[:, 0], -s.xyz[:, 1], s.xyz[:, 2]]), s.count)
+                              if i == 4 else box.slabs[i] for i in range(box.cpd)])
+    assert (rel_err(po.floor_f32(bad, R, eps, sl, ix, ce), ref) > 1e-3).mean() > 0.5
+
+
+def test_fp32_floor_exceeds_the_max_bar_on_the_64cubed_box():
+    """The precision study's box (N = 64^3, CPD 17, seed 20260064), all
+    262,144 sinks: FP32 pair terms formed as the batched kernel forms them,
+    summed exactly, already miss max <= 1e-5 outside softening-dominated
+    sinks, on sinks whose net force nearly cancels (condition number
+    ~100-1,000); the rms is ~2e-7.  So a max bar of 1e-5 against FP64 holds
+    only on sinks of bounded condition number, whatever the summation: the
+    GPU precision test holds the batched path to this floor."""
+    box = synth.uniform_box(64 ** 3, 17, seed=20260064)
+    eps, b2, _ = params(box)
+    parts = [po.slab_sinks(box, x) for x in range(box.cpd)]
+    sl, ix, ce = (np.concatenate([p[i] for p in parts]) for i in range(3))
+    ref, soft, mag = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce, with_soft=True, with_abs=True)
+    keep = soft <= 0.5 * np.linalg.norm(ref[:, :3], axis=1)
+    cond = mag / np.linalg.norm(ref[:, :3], axis=1)
+    rel = rel_err(po.floor_f32(box, R, eps, sl, ix, ce), ref)
+    st = summarize(rel, keep, tail=True)
+    assert st["max"] > 1e-5 and st["rms"] < 3e-7, st
+    worst = np.argmax(np.where(keep, rel, 0))
+    assert cond[worst] > 100
+    assert rel[keep & (cond <= 100)].max() < 1e-5
+    assert (rel[keep] / cond[keep]).max() < 1e-6
+
+
 def test_cpu_fallback_restatement_vs_fp64():
     box = small_uniform(cpd=9, ppc=15, seed=2)
     eps, b2, _ = params(box)
