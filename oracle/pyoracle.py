@@ -159,8 +159,78 @@ def execute_slab_cpu(box, R, kind, eps, x, y0=0, y1=None):
     return acc, int(di)
 
 
+def execute_slab_avx512(box, R, kind, eps, x, y0=0, y1=None):
+    """The same cell-pair loop on the explicit AVX-512 kernel (16 sources per
+    vector, rsqrt14 + one Newton step) -> ((n, 4) float32 acc, interactions)."""
+    lib = _load("libcpudirects.so")
+    lib.or_execute_slab_avx512.restype = C.c_uint64
+    a = _BoxArgs(box)
+    if y1 is None:
+        y1 = box.cpd
+    acc = np.zeros((box.slab(x).n, 4), np.float32)
+    di = lib.or_execute_slab_avx512(C.c_int(box.cpd), a.pos, _p(a.nslab), a.start, a.count, C.c_int(R),
+                                    C.c_int(kind_of(kind)), C.c_float(eps), C.c_int(x % box.cpd), C.c_int(y0),
+                                    C.c_int(y1), _p(acc))
+    return acc, int(di)
+
+
 def num_threads():
     return int(_load("libcpudirects.so").or_num_threads())
+
+
+def cpu_directs_codegen():
+    """What the compiler made of the two CPU-directs loops: per OpenMP body,
+    the widest vector register its FP32 arithmetic uses (16 lanes for zmm)
+    and whether the reciprocal square root is the 14-bit estimate
+    (vrsqrt14ps) refined in vector code.  None where objdump is missing."""
+    import shutil
+    import subprocess
+    if shutil.which("objdump") is None:
+        return None
+    txt = subprocess.run(["objdump", "-d", "--no-show-raw-insn", os.path.join(_HERE, "libcpudirects.so")],
+                         check=True, capture_output=True, text=True).stdout
+    out, cur = {}, None
+    for line in txt.splitlines():
+        if line.endswith(">:"):
+            name = line.split("<", 1)[1][:-2]
+            cur = name.split("._omp_fn")[0] if "._omp_fn" in name else None
+            if cur:
+                out.setdefault(cur, {"ops": {}})
+            continue
+        if cur and "ps" in line:
+            ins = line.split("\t")[-1].split()
+            if ins:
+                op, regs = ins[0], " ".join(ins[1:])
+                width = 16 if "%zmm" in regs else 8 if "%ymm" in regs else 4 if "%xmm" in regs else 0
+                if op.endswith("ps") and width:
+                    out[cur]["ops"][op] = max(out[cur]["ops"].get(op, 0), width)
+    for v in out.values():
+        ops = v.pop("ops")
+        arith = [w for o, w in ops.items() if o.startswith(("vfmadd", "vfnmadd", "vmul", "vsub", "vadd"))]
+        v["vector_lanes"] = max(arith) if arith else 1
+        v["rsqrt"] = "vrsqrt14ps" if "vrsqrt14ps" in ops else ("vsqrtps" if "vsqrtps" in ops else "scalar")
+        v["rsqrt_lanes"] = ops.get("vrsqrt14ps", 0)
+    return out
+
+
+def cpu_info():
+    """The host CPU: model name, logical CPUs, and whether it has avx512f."""
+    model, flags, ncpu = "unknown", set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "processor":
+                    ncpu += 1
+                elif k == "model name" and model == "unknown":
+                    model = v.strip()
+                elif k == "flags" and not flags:
+                    flags = set(v.split())
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": ncpu, "avx512f": "avx512f" in flags,
+            "isa": sorted(f for f in flags if f.startswith("avx512"))}
 
 
 # ---- pencil-level restatements ---------------------------------------------

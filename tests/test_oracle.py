@@ -301,6 +301,43 @@ def test_cpu_fallback_restatement_vs_fp64():
         assert np.all(acc[:, 3] == 0)  # the CPU fallback computes no density
 
 
+def test_cpu_avx512_form_vs_fp64_and_autovectorised():
+    """The explicit AVX-512 kernel (16 sources per vector, masked remainder,
+    rsqrt14 + one Newton step) on the same cell-pair loop: the same
+    interaction count, FP64-oracle errors of the FP32 size (the bare 14-bit
+    estimate would give ~1e-4), and agreement with the auto-vectorised
+    production form; also for the Plummer form and cells whose counts are not
+    multiples of 16."""
+    box = small_uniform(cpd=9, ppc=15, seed=2)
+    eps, b2, _ = params(box)
+    for x in (0, 5):
+        acc_v, di_v = po.execute_slab_avx512(box, R, "single_spline", eps, x)
+        acc_c, di_c = po.execute_slab_cpu(box, R, "single_spline", eps, x)
+        assert di_v == di_c == po.calc_di(box, R, x)
+        sl, ix, ce = po.slab_sinks(box, x)
+        ref = po.direct_f64(box, R, "single_spline", eps, b2, sl, ix, ce)
+        st = summarize(rel_err(acc_v, ref))
+        assert st["rms"] < 3e-6, st
+        assert summarize(rel_err(acc_v, acc_c))["rms"] < 3e-6
+        assert np.all(acc_v[:, 3] == 0)
+    eps_p = (0.3 / box.cpd) ** 2
+    acc_v, _ = po.execute_slab_avx512(box, R, "plummer", eps_p, 3)
+    sl, ix, ce = po.slab_sinks(box, 3)
+    ref = po.direct_f64(box, R, "plummer", eps_p, b2, sl, ix, ce)
+    assert summarize(rel_err(acc_v, ref))["rms"] < 3e-6
+
+
+def test_cpu_directs_are_16_wide_with_refined_rsqrt14():
+    """Both CPU loops compile to 16-lane (zmm) FP32 arithmetic with the
+    vrsqrt14ps estimate, as the reference's -Ofast -mprefer-vector-width=512
+    build of the same loop does (common.mk.in:97-109)."""
+    cg = po.cpu_directs_codegen()
+    if cg is None:
+        pytest.skip("objdump not available")
+    for fn in ("or_execute_slab_cpu", "or_execute_slab_avx512"):
+        assert cg[fn]["vector_lanes"] == 16 and cg[fn]["rsqrt"] == "vrsqrt14ps" and cg[fn]["rsqrt_lanes"] == 16, cg
+
+
 def test_pencil_restatement_matches_cell_pair():
     """Pencil-structured (CPUExecute) and cell-pair (ExecuteSlabCPU) give the
     same physics: the pencil decomposition covers each neighbour cell once."""
