@@ -16,6 +16,8 @@ Workloads:
   * ``clustered_slabs`` the same recipe at production size (CPD 255, ~1e9
     particles), realised only in the slabs a run needs.
 """
+import os
+
 import numpy as np
 
 CELLINFO_DTYPE = np.dtype(
@@ -231,8 +233,19 @@ def _in_slabs(x_lo, x_hi, cpd, wanted):
     return bool(wanted[np.arange(c0, c1 + 1) % cpd].any())
 
 
+def _in_slabs_many(x_lo, x_hi, cpd, wanted):
+    """_in_slabs over arrays of intervals, each at most 2 cpd wide and within
+    one box length of the box."""
+    c0 = np.floor((np.asarray(x_lo) + 0.5) * cpd).astype(np.int64)
+    c1 = np.floor((np.asarray(x_hi) + 0.5) * cpd).astype(np.int64)
+    cum = np.concatenate([[0], np.cumsum(np.tile(wanted, 4))])     # wanted counts over [-cpd, 3 cpd)
+    n = cum[np.clip(c1, -cpd, 3 * cpd - 1) + cpd + 1] - cum[np.clip(c0, -cpd, 3 * cpd - 1) + cpd]
+    return np.where(c1 - c0 + 1 >= cpd, bool(wanted.any()), n > 0)
+
+
 def clustered_slabs(n1d, cpd, seed, slabs, grf_mesh=256, halos_per_cell=2000 / 69 ** 3, halo_frac=0.3,
-                    rmin=0.02, rmax=0.3, mass_slope=1.9, mass_range=4.0, rtrunc=10.0, lattice_frac=0.7):
+                    rmin=0.02, rmax=0.3, mass_slope=1.9, mass_range=4.0, rtrunc=10.0, lattice_frac=0.7,
+                    mesh=None):
     """The clustered recipe of ``clustered_box`` at any size, realised only in
     the listed slabs (every other slab of the returned Box is empty).
 
@@ -247,13 +260,22 @@ def clustered_slabs(n1d, cpd, seed, slabs, grf_mesh=256, halos_per_cell=2000 / 6
         [M0, mass_range M0]), centres and scale radii (log-uniform in
         [rmin, rmax] cells) come from seed (seed, 2), and halo h's particles
         from seed (seed, 3, h).
-    The nominal particle count (for the mean spacing) is n1d^3."""
+    The nominal particle count (for the mean spacing) is n1d^3.  ``mesh``
+    passes in the GRF mesh of these arguments when the caller already has it."""
     w = 1.0 / cpd
     wanted = np.zeros(cpd, bool)
     wanted[np.asarray(list(slabs)) % cpd] = True
-    mesh = _grf_mesh(grf_mesh, 0.5 * w, np.random.default_rng([seed, 0]))
+    if mesh is None:
+        mesh = _grf_mesh(grf_mesh, 0.5 * w, np.random.default_rng([seed, 0]))
     maxd = float(np.abs(mesh).max())
     parts = []
+
+    def wanted_only(p):
+        # wrapped into the box; only the particles of wanted slabs are kept
+        p = p - np.floor(p + 0.5)
+        sx = np.clip(np.floor((p[:, 0] + 0.5) * cpd).astype(np.int64), 0, cpd - 1)
+        return p[wanted[sx]]
+
     ax = (np.arange(n1d) + 0.5) / n1d - 0.5
     gy, gz = np.meshgrid(ax, ax, indexing="ij")
     gy, gz = gy.ravel(), gz.ravel()
@@ -263,7 +285,7 @@ def clustered_slabs(n1d, cpd, seed, slabs, grf_mesh=256, halos_per_cell=2000 / 6
         rng = np.random.default_rng([seed, 1, i])
         keep = rng.random(n1d * n1d) < lattice_frac
         d = _plane_displacement(mesh, ax[i], n1d).reshape(-1, 3)[keep]
-        parts.append(np.stack([ax[i] + d[:, 0], gy[keep] + d[:, 1], gz[keep] + d[:, 2]], axis=1))
+        parts.append(wanted_only(np.stack([ax[i] + d[:, 0], gy[keep] + d[:, 1], gz[keep] + d[:, 2]], axis=1)))
     rng = np.random.default_rng([seed, 2])
     n_halos = max(1, int(round(halos_per_cell * cpd ** 3)))
     n_halo_parts = int(round(halo_frac * n1d ** 3))
@@ -275,21 +297,54 @@ def clustered_slabs(n1d, cpd, seed, slabs, grf_mesh=256, halos_per_cell=2000 / 6
     centres = rng.random((n_halos, 3)) - 0.5
     radii = np.exp(rng.uniform(np.log(rmin), np.log(rmax), n_halos)) * w
     xmax = rtrunc ** 3 / (rtrunc ** 2 + 1.0) ** 1.5
-    for h in range(n_halos):
-        ext = rtrunc * radii[h]
-        if nh[h] == 0 or not _in_slabs(centres[h, 0] - ext, centres[h, 0] + ext, cpd, wanted):
-            continue
+    ext = rtrunc * radii
+    near = _in_slabs_many(centres[:, 0] - ext, centres[:, 0] + ext, cpd, wanted) & (nh > 0)
+    for h in np.flatnonzero(near):
         hr = np.random.default_rng([seed, 3, h])
         X = hr.random(nh[h]) * xmax
         r = radii[h] / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
         v = hr.standard_normal((nh[h], 3))
         v /= np.linalg.norm(v, axis=1)[:, None]
-        parts.append(centres[h] + r[:, None] * v)
+        parts.append(wanted_only(centres[h] + r[:, None] * v))
     pos = np.concatenate(parts) if parts else np.zeros((0, 3))
-    pos = pos - np.floor(pos + 0.5)
-    sx = np.clip(np.floor((pos[:, 0] + 0.5) * cpd).astype(np.int64), 0, cpd - 1)
-    pos = pos[wanted[sx]]
     box = box_from_global(pos, cpd, f"clustered n1d={n1d} cpd={cpd} seed={seed} slabs {int(wanted.sum())}/{cpd}")
+    box.N = n1d ** 3
+    return box
+
+
+_PAR_MESH = None  # the GRF mesh a forked generator pool inherits
+
+
+def _clustered_chunk(job):
+    n1d, cpd, seed, chunk, kw = job
+    box = clustered_slabs(n1d, cpd, seed, chunk, mesh=_PAR_MESH, **kw)
+    return [(x % cpd, box.slab(x)) for x in chunk]
+
+
+def clustered_slabs_parallel(n1d, cpd, seed, slabs, workers=None, chunk=4, **kw):
+    """``clustered_slabs`` realised by a pool of forked processes, ``chunk``
+    slabs per job: the same particles slab for slab (each slab's particles
+    depend only on the seed), in a fraction of the time at production size
+    (CPD 255, ~1e9 particles).  The GRF mesh is computed once, here, and
+    inherited by the workers.  Forks: call it before the process touches a
+    GPU."""
+    import multiprocessing as mp
+    global _PAR_MESH
+    want = sorted({x % cpd for x in slabs})
+    grf = kw.pop("grf_mesh", 256)
+    _PAR_MESH = _grf_mesh(grf, 0.5 / cpd, np.random.default_rng([seed, 0]))
+    kw["grf_mesh"] = grf
+    jobs = [(n1d, cpd, seed, want[i:i + chunk], kw) for i in range(0, len(want), chunk)]
+    workers = workers or min(len(jobs), 16, os.cpu_count() or 1)
+    out = [Slab(np.zeros(0, np.float32), np.zeros(cpd * cpd, np.int32)) for _ in range(cpd)]
+    try:
+        with mp.get_context("fork").Pool(workers) as pool:
+            for part in pool.imap_unordered(_clustered_chunk, jobs):
+                for x, sl in part:
+                    out[x] = sl
+    finally:
+        _PAR_MESH = None
+    box = Box(cpd, out, f"clustered n1d={n1d} cpd={cpd} seed={seed} slabs {len(want)}/{cpd}")
     box.N = n1d ** 3
     return box
 

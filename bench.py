@@ -6,11 +6,12 @@ equivalent length 1/40 of the mean spacing, FOF density radius 0.25 spacing.
 The whole box (PosXYZ slabs and float4 acceleration slabs) is resident in HBM;
 a step is one near-field evaluation of the whole box: every slab's
 SetInteractionCollection submitted through the C ABI (device-resident mode),
-gather + direct kernel + co-add on the GPU.  The SIC plans are built once from
-the cellinfo before timing (host planning time is reported separately); with
---planner device every step submits the slabs through nf_submit_slab, which
-plans them on the GPU inside the timed region.  5 timed steps by default; the
-line reports their total and their median.
+gather + direct kernel + co-add on the GPU.  Every step plans each slab's SIC
+from the cellinfo (nf_plan_sic on the host) and submits it at once, so planning
+overlaps the GPU work already queued, as ExecuteSlabGPU does; with --planner
+device every step submits the slabs through nf_submit_slab, which plans them on
+the GPU.  5 timed steps by default; the line reports their total and their
+median.
 
 Prints one JSON line (rank 0).  Unit of work: unpadded pair interactions
 including self pairs (DirectTotal); 20 FLOP per interaction for % of peak.
@@ -74,9 +75,12 @@ def main():
     ap.add_argument("--graphs", type=int, default=None,
                     help="capture each group's kernels in a HIP graph (default: on for slab streams)")
     ap.add_argument("--paths", default="plain,batched")
-    ap.add_argument("--planner", default="host", choices=["host", "device"],
-                    help="host: SICs planned by nf_plan_sic before the timed steps; device: each step submits "
-                         "the slabs with nf_submit_slab, which plans them on the GPU inside the timed region")
+    ap.add_argument("--planner", default="host", choices=["host", "device", "host-once"],
+                    help="host: each step plans every slab's SIC with nf_plan_sic and submits it at once, so "
+                         "planning the next SIC overlaps the GPU work of the ones before (ExecuteSlabGPU's "
+                         "order); device: each step submits the slabs with nf_submit_slab, which plans them "
+                         "on the GPU; host-once: the SICs are planned once before timing and resubmitted "
+                         "every step (a comparison only: real steps re-plan, particles move)")
     ap.add_argument("--bpd", type=int, default=4, help="worker streams (DirectBPD)")
     ap.add_argument("--sinks-per-lane", type=int, default=0, help="0: the autotuner picks per occupancy histogram")
     ap.add_argument("--max-group", type=int, default=8, help="SICs per persistent launch (batched path)")
@@ -145,6 +149,9 @@ def main():
                 if args.planner == "device":
                     tasks = [drv.submit_slab(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr,
                                              mem=nfm.NF_MEM_DEVICE) for x in range(cpd)]
+                elif args.planner == "host":
+                    tasks = [drv.submit(drv.plan(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr,
+                                                 mem=nfm.NF_MEM_DEVICE)) for x in range(cpd)]
                 else:
                     tasks = [drv.submit(s) for s in sics]
                 st = []
@@ -437,7 +444,13 @@ def run_stream(args):
 
         def step():
             drv.stream_begin()
-            if args.planner == "device":
+            if args.planner == "host":
+                tasks = [drv.submit(drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST,
+                                             pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data
+                                                                                for c in range(2 * R + 1)],
+                                             pos_layout=nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ,
+                                             kick=kicks.get(x))) for x in sinks]
+            elif args.planner == "device":
                 tasks = [drv.submit_slab(box, x, acc_ptr=acc[x].ctypes.data,
                                          pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data
                                                                             for c in range(2 * R + 1)],

@@ -69,3 +69,17 @@ def test_reference_ewald_fixture():
         sic = nfm.SetInteractionCollection(box, x, 0, box.cpd, R, eps, b2, "single_spline")
         assert sic.desc.DirectTotal == po.calc_di(box, R, x)
 
+
+
+def test_clustered_slabs_parallel_matches_serial():
+    """The forked-pool generator gives the serial generator's slabs, bit for
+    bit, whatever the chunking (each slab depends only on the seed)."""
+    want = [-1, 0, 1, 2, 5, 6]
+    ref = synth.clustered_slabs(120, 31, 77, want, **KW)
+    for chunk in (1, 4):
+        par = synth.clustered_slabs_parallel(120, 31, 77, want, workers=3, chunk=chunk, **KW)
+        assert par.N == ref.N == 120 ** 3
+        for x in range(31):
+            np.testing.assert_array_equal(par.slab(x).count, ref.slab(x).count)
+            np.testing.assert_array_equal(par.slab(x).pos, ref.slab(x).pos)
+        assert par.slab(2).n > 0 and par.slab(10).n == 0
