@@ -59,7 +59,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -88,13 +88,26 @@ def main():
                     help="pair interactions after which a launch group takes no more SICs (0: the library's default)")
     ap.add_argument("--gpu-streams", type=int, default=1,
                     help="1: workers share one HIP stream (kernels back to back), 0: one per worker")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="whole-box workloads: skip the other workloads' lines (clustered 256^3 plain and "
+                         "batched, the CPD 255 slab stream) that the default run adds under 'configs'")
+    ap.add_argument("--config-steps", type=int, default=5, help="timed steps of each 'configs' workload")
+    ap.add_argument("--config-stream-slabs", type=int, default=24,
+                    help="sink slabs per pass of the slab stream under 'configs'")
+    ap.add_argument("--error-sample", type=float, default=0.01,
+                    help="fraction of sinks checked against the FP64 oracle in the CPU leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU leg (the CPU baseline and the FP64-oracle error checks)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device-to-device twice (counter calibration) and exit")
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="only print the HBM traffic table from rocprofv3 --pmc databases under DIR and exit")
-    args = ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def main():
+    args = parse_args()
     if args.pmc_summary:
         print_pmc_summary(args.pmc_summary, args.workload, args.calibrate_copy_mib or 1024)
         return
@@ -106,44 +119,100 @@ def main():
               "(the path would shard each slab's y-rows across GPUs, with a 2R-row source halo)",
               file=sys.stderr)
         sys.exit(2)
+    R = 2
     if args.workload in STREAM_WORKLOADS:
-        run_stream(args)
+        box = build_stream(args.workload, args.stream_slabs)
+        res = run_stream(args, box, args.workload, args.stream_slabs, args.steps, args.warmup)
+        line = stream_line(args, res)
+        if not args.no_cpu and not args.fuse_kick:
+            sample = {x: res["acc"][x][:2000] for x in (res["sinks"][0], res["sinks"][len(res["sinks"]) // 2])}
+            line["cpu_baseline"] = cpu_baseline(box, R, res["soft_len"], args.cpu_seconds,
+                                                {"batched": {"acc_sample": sample}}, slabs=res["sinks"])
+            line["accuracy_vs_fp64"] = error_check(box, R, res["soft_len"], {"batched": res["acc"]},
+                                                   args.error_sample, 255, slabs=res["sinks"])
+        print(json.dumps(line), flush=True)
         return
 
+    # The slab stream of the 'configs' lines is realised first: its generator
+    # forks worker processes, which must happen before this process touches
+    # the GPU.
+    stream_box = None if args.no_configs else build_stream("slabstream255", args.config_stream_slabs)
     build, cpd, desc = WORKLOADS[args.workload]
     t0 = time.time()
     box = build()
     log(f"[bench] built {box.name}: N={box.N} in {time.time() - t0:.1f}s")
+    paths = [p for p in args.paths.split(",") if p]
+    res = run_resident(args, box, cpd, paths, args.steps, args.warmup, keep_acc=not args.no_cpu)
+    line = resident_line(args, box, cpd, desc, res)
+    extra = {}
+    if not args.no_configs:
+        cbuild, ccpd, cdesc = WORKLOADS["clustered256"]
+        cbox = cbuild()
+        log(f"[bench] built {cbox.name}: N={cbox.N}")
+        cres = run_resident(args, cbox, ccpd, ["plain", "batched"], args.config_steps, 1,
+                            keep_acc=not args.no_cpu)
+        cl = {"workload": cdesc, "N": cbox.N, "cpd": ccpd, "steps": args.config_steps, "warmup": 1,
+              "interactions_per_step": cres["plain"]["direct_total"],
+              "max_cell_occupancy": int(cbox.counts().max()),
+              "paths": {p: summary_of(d) for p, d in cres.items()},
+              "batched_over_plain": cres["batched"]["interactions_per_s"] / cres["plain"]["interactions_per_s"],
+              "batched_over_plain_kernel": cres["batched"]["kernel_interactions_per_s"]
+              / cres["plain"]["kernel_interactions_per_s"]}
+        sres = run_stream(args, stream_box, "slabstream255", args.config_stream_slabs, args.config_steps, 1)
+        sl = stream_line(args, sres)
+        st = {"workload": sl["config"]["workload"], "steps": args.config_steps, "warmup": 1, **sl["config"],
+              **sl["end_to_end"], "autotune": sl["autotune"]}
+        line["configs"] = {"clustered256": cl, "slabstream255": st}
+        extra = {"clustered256": (cbox, cres), "slabstream255": (stream_box, sres)}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(box, R, box.mean_spacing() / 40.0, args.cpu_seconds, res)
+        line["accuracy_vs_fp64"] = error_check(box, R, box.mean_spacing() / 40.0,
+                                               {p: d["acc"] for p, d in res.items()}, args.error_sample, 512)
+        if extra:
+            cbox, cres = extra["clustered256"]
+            line["configs"]["clustered256"]["accuracy_vs_fp64"] = error_check(
+                cbox, R, cbox.mean_spacing() / 40.0, {p: d["acc"] for p, d in cres.items()}, args.error_sample, 256)
+            sbox, sres = extra["slabstream255"]
+            line["configs"]["slabstream255"]["accuracy_vs_fp64"] = error_check(
+                sbox, R, sres["soft_len"], {"batched": sres["acc"]}, args.error_sample, 255, slabs=sres["sinks"])
+    print(json.dumps(line), flush=True)
+
+
+def run_resident(args, box, cpd, paths, steps, warmup, keep_acc=False):
+    """Whole box resident in HBM; per path, `warmup` untimed and `steps` timed
+    steps, each one near-field evaluation of every slab (planning included,
+    per --planner).  Returns {path: results}, with each path's accelerations
+    on the host under "acc" when keep_acc."""
+    R = 2
     ell = box.mean_spacing()
     soft_len = ell / 40.0
     b2 = synth.fof_b2(ell)
-    R = 2
-
-    # Resident slabs in HBM.
     pos_d = [nfm.DeviceArray(host=s.pos) for s in box.slabs]
     acc_d = [nfm.DeviceArray(nbytes=16 * s.n) for s in box.slabs]
     for a in acc_d:
         a.zero()
     nfm.device_synchronize()
     log(f"[bench] uploaded {sum(s.n for s in box.slabs) * 12 / 1e9:.2f} GB of positions")
-
     results = {}
-    paths = [p for p in args.paths.split(",") if p]
     for path in paths:
         with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path=path, buffers_per_device=args.bpd,
                                  max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
                                  gpu_streams=args.gpu_streams, max_group=args.max_group,
                                  group_interactions=int(args.group_interactions),
                                  use_graphs=args.graphs or 0) as drv:
-            tp = time.time()
-            sics = []
             ptrs_of = {x: [pos_d[(x - R + c) % cpd].ptr for c in range(2 * R + 1)] for x in range(cpd)}
-            for x in range(cpd):
-                sics.append(drv.plan(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr, mem=nfm.NF_MEM_DEVICE))
+            tp = time.time()
+            sics = [drv.plan(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr, mem=nfm.NF_MEM_DEVICE)
+                    for x in range(cpd)]
             plan_s = time.time() - tp
             direct_total = sum(int(s.desc.DirectTotal) for s in sics)
             padded_total = sum(int(s.desc.PaddedDirectTotal) for s in sics)
             log(f"[bench] {path}: planned {cpd} SICs in {plan_s:.2f}s, DirectTotal={direct_total:.4e}")
+            sic_sizes = dict(padded_sinks=float(np.mean([s.desc.PaddedSinkTotal for s in sics])),
+                             padded_sources=float(np.mean([s.desc.PaddedSourceTotal for s in sics])))
+            per_slab_di = [int(s.desc.DirectTotal) for s in sics]
+            if args.planner != "host-once":
+                sics = None  # each step plans its own
 
             def step():
                 if args.planner == "device":
@@ -161,26 +230,26 @@ def main():
                     t.release()
                 return st
 
-            for i in range(args.warmup):
+            for i in range(warmup):
                 step()
-                log(f"[bench] {path}: warmup {i + 1}/{args.warmup} done")
+                log(f"[bench] {path}: warmup {i + 1}/{warmup} done")
             nfm.device_synchronize()
             t1 = time.perf_counter()
             kstats, step_walls = [], []
-            for i in range(args.steps):
+            for i in range(steps):
                 ts = time.perf_counter()
                 kstats.extend(step())  # returns once every task of the step has completed
                 step_walls.append(time.perf_counter() - ts)
-                log(f"[bench] {path}: step {i + 1}/{args.steps} done")
+                log(f"[bench] {path}: step {i + 1}/{steps} done")
             nfm.device_synchronize()
             wall = time.perf_counter() - t1
             kern = np.array([s["direct_kernel_s"] for s in kstats])      # each task's share of its launch
             tot = np.array([s["kernel_s"] for s in kstats])
             launches = float(sum(1.0 / s["group_size"] for s in kstats))
-            per_slab_di = np.array([int(s.desc.DirectTotal) for s in sics] * args.steps, dtype=np.float64)
+            di_all = np.array(per_slab_di * steps, dtype=np.float64)
             # Per direct-kernel instantiation (tile shape): launches, time and work.
             shapes = {}
-            for st_, di_ in zip(kstats, per_slab_di):
+            for st_, di_ in zip(kstats, di_all):
                 e = shapes.setdefault(kernel_name(path, st_["sinks_per_lane"], st_["waves_per_eu"]),
                                       dict(launches=0.0, kernel_s=0.0, interactions=0.0))
                 e["launches"] += 1.0 / st_["group_size"]
@@ -190,29 +259,40 @@ def main():
                 e["avg_ms"] = 1e3 * e["kernel_s"] / e["launches"]
                 e["interactions_per_launch"] = e["interactions"] / e["launches"]
             results[path] = dict(
-                wall_s=wall, ms_per_step=1e3 * wall / args.steps,
+                wall_s=wall, ms_per_step=1e3 * wall / steps,
                 median_step_ms=1e3 * float(np.median(step_walls)),
-                interactions_per_s=direct_total * args.steps / wall,
-                direct_kernel_s_per_step=float(kern.sum()) / args.steps,
-                device_s_per_step=float(tot.sum()) / args.steps,
-                kernel_interactions_per_s=float(per_slab_di.sum() / kern.sum()),
+                interactions_per_s=direct_total * steps / wall,
+                direct_kernel_s_per_step=float(kern.sum()) / steps,
+                device_s_per_step=float(tot.sum()) / steps,
+                kernel_interactions_per_s=float(di_all.sum() / kern.sum()),
                 kernel_avg_ms=float(kern.sum() / launches * 1e3),
-                launches_per_step=launches / args.steps,
-                interactions_per_launch=float(per_slab_di.sum() / launches),
-                direct_total=direct_total, padded_direct_total=padded_total, plan_s=plan_s,
-                device_plan_s_per_step=float(sum(s["plan_s"] for s in kstats)) / args.steps,
-                kernels=shapes, n_sics=len(sics),
-                sic_sizes=dict(padded_sinks=float(np.mean([s.desc.PaddedSinkTotal for s in sics])),
-                               padded_sources=float(np.mean([s.desc.PaddedSourceTotal for s in sics]))),
+                launches_per_step=launches / steps,
+                interactions_per_launch=float(di_all.sum() / launches),
+                direct_total=direct_total, padded_direct_total=padded_total, plan_s_before_timing=plan_s,
+                planner=args.planner,
+                device_plan_s_per_step=float(sum(s["plan_s"] for s in kstats)) / steps,
+                kernels=shapes, n_sics=cpd, sic_sizes=sic_sizes,
             )
-            results[path]["acc_sample"] = {x: acc_d[x].to_host(np.float32, (box.slab(x).n, 4))[:2000]
-                                           for x in (0, cpd // 2)}
+            if keep_acc:
+                results[path]["acc"] = {x: acc_d[x].to_host(np.float32, (box.slab(x).n, 4)) for x in range(cpd)}
+                results[path]["acc_sample"] = {x: results[path]["acc"][x][:2000] for x in (0, cpd // 2)}
             if path == "batched":
                 results[path]["autotune"] = [{k: e[k] for k in ("sinks_per_lane", "waves_per_cu", "uses")}
                                              for e in drv.autotune_table()]
             log(f"[bench] {path}: {results[path]['interactions_per_s']:.4e} interactions/s wall, "
                 f"kernel {results[path]['kernel_interactions_per_s']:.4e}/s")
+    for d in pos_d + acc_d:
+        d.free()
+    return results
 
+
+def summary_of(d):
+    """A path's results without the accelerations."""
+    return {k: v for k, v in d.items() if k not in ("acc", "acc_sample")}
+
+
+def resident_line(args, box, cpd, desc, results):
+    R = 2
     best = max(results, key=lambda p: results[p]["interactions_per_s"])
     r = results[best]
     # The dominant kernel: the direct-kernel instantiation with the most time.
@@ -234,7 +314,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "N": box.N, "cpd": cpd, "nfradius": R, "path": best,
                    "softening": "single_spline", "interactions_per_step": r["direct_total"],
-                   "planner": args.planner},
+                   "planner": args.planner, "graphs": args.graphs or 0},
         "pct_fp32_peak": 100.0 * FLOP_PER_INTERACTION * r["interactions_per_s"] / (PEAK_FP32_TFLOPS * 1e12),
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": traffic,
@@ -243,14 +323,11 @@ def main():
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(r, kd),
                      "traffic_source": traffic_src},
-        "paths": {p: {k: v for k, v in d.items() if k != "acc_sample"} for p, d in results.items()},
+        "paths": {p: summary_of(d) for p, d in results.items()},
     }
     if "plain" in results and "batched" in results:
         line["batched_over_plain"] = results["batched"]["interactions_per_s"] / results["plain"]["interactions_per_s"]
-
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(box, R, soft_len, args.cpu_seconds, results)
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def kernel_name(path, sinks_per_lane, waves_per_eu=0, softening=1):
@@ -388,18 +465,29 @@ def print_pmc_summary(prof_dir, workload, copy_mib):
         print(f"| {k} | {max(nf, nw)} | {fa:.1f} | {wa:.1f} | {b / 1e6:.2f} | {b:.0f} |")
 
 
-def run_stream(args):
+def build_stream(workload, nsinks):
+    """The slabs a pass over sink slabs 0 .. nsinks-1 needs (R more on each
+    side), realised by the forked-pool generator."""
+    _, cpd, n1d, _ = STREAM_WORKLOADS[workload]
+    R = 2
+    t0 = time.time()
+    box = synth.clustered_slabs_parallel(n1d, cpd, 20260255, range(-R, nsinks + R))
+    log(f"[bench] built {box.name}: {sum(s.n for s in box.slabs)} particles in {time.time() - t0:.1f}s")
+    return box
+
+
+def run_stream(args, box, workload, nsinks, steps, warmup, options=True):
     """A slab stream: PosXYZ slabs and acceleration slabs in host memory, the
     sink slabs submitted in order inside one stream session (each source slab
     uploaded once while it is in the 2R+1 window), every launch group captured
-    in a HIP graph, tiles autotuned.  A step is one pass over the sink slabs;
-    the timed region includes every host-device transfer."""
-    build, cpd, n1d, desc = STREAM_WORKLOADS[args.workload]
+    in a HIP graph, tiles autotuned.  A step is one pass over the sink slabs,
+    each SIC planned in the step (per --planner); the timed region includes
+    every host-device transfer.  options=False ignores --pinned, --aos and
+    --fuse-kick (pageable PosXYZ slabs, plain near-field output)."""
+    _, cpd, n1d, desc = STREAM_WORKLOADS[workload]
     R = 2
-    sinks = list(range(args.stream_slabs))
-    t0 = time.time()
-    box = build(range(-R, args.stream_slabs + R))
-    log(f"[bench] built {box.name}: {sum(s.n for s in box.slabs)} particles in {time.time() - t0:.1f}s")
+    pinned, use_aos, fuse = (args.pinned, args.aos, args.fuse_kick) if options else (False, False, False)
+    sinks = list(range(nsinks))
     ell = box.mean_spacing()
     soft_len = ell / 40.0
     b2 = synth.fof_b2(ell)
@@ -407,22 +495,22 @@ def run_stream(args):
     keep = []
 
     def host_array(shape, like=None):
-        if not args.pinned:
+        if not pinned:
             return np.zeros(shape, np.float32) if like is None else like
         pa = nfm.PinnedArray(shape, np.float32)
         keep.append(pa)
         pa.array[:] = 0 if like is None else like
         return pa.array
 
-    if args.pinned:
+    if pinned:
         box = synth.Box(cpd, [synth.Slab(host_array(s.pos.shape, s.pos), s.count) for s in box.slabs], box.name)
         box.N = n1d ** 3
     acc = {x: host_array((box.slab(x).n, 4)) for x in sinks}
     # posstruct copies of the slabs (what the reference's PosSlab holds before TransposePos)
     aos = {x: host_array((box.slab(x).n * 3,), box.slab(x).xyz.reshape(-1)) for x in range(-R, len(sinks) + R)} \
-        if args.aos else None
+        if use_aos else None
     kicks = {}
-    if args.fuse_kick:
+    if fuse:
         rng = np.random.default_rng(7)
         _, e_int = synth.eps_param("single_spline", soft_len)
         for x in sinks:
@@ -430,33 +518,33 @@ def run_stream(args):
             kicks[x] = nfm.nf_kick(FarAccSlab=far.ctypes.data, rescale=-3.0 * 0.3 / (8.0 * np.pi * box.N),
                                    inv_eps3=1.0 / e_int ** 3, density_rad2=float(b2))
             keep.append(far)
+
+    def pos_ptrs(x):
+        return None if aos is None else [aos[x - R + c].ctypes.data for c in range(2 * R + 1)]
+
+    layout = nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ
     with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path="batched", buffers_per_device=args.bpd,
                              max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
                              gpu_streams=args.gpu_streams, max_group=args.max_group, use_graphs=graphs,
                              group_interactions=int(args.group_interactions)) as drv:
         tp = time.time()
-        sics = [drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST,
-                         pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data for c in range(2 * R + 1)],
-                         pos_layout=nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ, kick=kicks.get(x))
-                for x in sinks]
+        sics = [drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST, pos_ptrs=pos_ptrs(x),
+                         pos_layout=layout, kick=kicks.get(x)) for x in sinks]
         plan_s = time.time() - tp
         direct_total = sum(int(s.desc.DirectTotal) for s in sics)
+        if args.planner != "host-once":
+            sics = None  # each step plans its own
 
         def step():
             drv.stream_begin()
             if args.planner == "host":
                 tasks = [drv.submit(drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST,
-                                             pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data
-                                                                                for c in range(2 * R + 1)],
-                                             pos_layout=nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ,
-                                             kick=kicks.get(x))) for x in sinks]
+                                             pos_ptrs=pos_ptrs(x), pos_layout=layout, kick=kicks.get(x)))
+                         for x in sinks]
             elif args.planner == "device":
-                tasks = [drv.submit_slab(box, x, acc_ptr=acc[x].ctypes.data,
-                                         pos_ptrs=None if aos is None else [aos[x - R + c].ctypes.data
-                                                                            for c in range(2 * R + 1)],
-                                         mem=nfm.NF_MEM_HOST,
-                                         pos_layout=nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ,
-                                         kick=kicks.get(x)) for x in sinks]
+                tasks = [drv.submit_slab(box, x, acc_ptr=acc[x].ctypes.data, pos_ptrs=pos_ptrs(x),
+                                         mem=nfm.NF_MEM_HOST, pos_layout=layout, kick=kicks.get(x))
+                         for x in sinks]
             else:
                 tasks = [drv.submit(s) for s in sics]
             st = []
@@ -467,64 +555,69 @@ def run_stream(args):
             drv.stream_end()
             return st
 
-        for i in range(args.warmup):
+        for i in range(warmup):
             step()
-            log(f"[bench] stream: warmup {i + 1}/{args.warmup} done")
+            log(f"[bench] stream: warmup {i + 1}/{warmup} done")
         walls, stats = [], []
-        for i in range(args.steps):
+        for i in range(steps):
             t1 = time.perf_counter()
             stats.extend(step())
             walls.append(time.perf_counter() - t1)
-            log(f"[bench] stream: step {i + 1}/{args.steps} done ({walls[-1] * 1e3:.1f} ms)")
+            log(f"[bench] stream: step {i + 1}/{steps} done ({walls[-1] * 1e3:.1f} ms)")
         table = drv.autotune_table()
-    wall = float(sum(walls))
+    return dict(desc=desc, cpd=cpd, n1d=n1d, sinks=sinks, box=box, acc=acc, soft_len=soft_len, walls=walls,
+                stats=stats, steps=steps, warmup=warmup, direct_total=direct_total, plan_s=plan_s,
+                table=table, graphs=graphs, pinned=pinned, aos=use_aos, fuse_kick=fuse, keep=keep)
+
+
+def stream_line(args, res):
+    R = 2
+    box, stats, steps = res["box"], res["stats"], res["steps"]
+    wall = float(sum(res["walls"]))
     up = sum(x["upload_s"] for x in stats)
     down = sum(x["download_s"] for x in stats)
     busy = sum(x["kernel_s"] for x in stats)
     kern = sum(x["direct_kernel_s"] for x in stats)
     xfer = up + down
     hidden = min(1.0, max(0.0, (busy + xfer - wall) / xfer)) if xfer > 0 else None
-    di = direct_total * args.steps
-    line = {
+    di = res["direct_total"] * steps
+    nsl = len(res["sinks"])
+    return {
         "metric": METRIC,
         "value": di / wall,
         "unit": "interactions/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "steps": steps,
+        "warmup": res["warmup"],
+        "ms_per_step": 1e3 * wall / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": desc, "cpd": cpd, "N_nominal": n1d ** 3, "sink_slabs": len(sinks),
-                   "particles_in_window": int(sum(box.slab(x).n for x in range(-R, len(sinks) + R))),
-                   "interactions_per_step": direct_total, "path": "batched", "graphs": graphs,
-                   "softening": "single_spline", "mem": "host pinned" if args.pinned else "host pageable",
-                   "positions": "posstruct (AoS)" if args.aos else "PosXYZ", "fused_kick": bool(args.fuse_kick),
+        "config": {"workload": res["desc"], "cpd": res["cpd"], "N_nominal": res["n1d"] ** 3, "sink_slabs": nsl,
+                   "particles_in_window": int(sum(box.slab(x).n for x in range(-R, nsl + R))),
+                   "interactions_per_step": res["direct_total"], "path": "batched", "graphs": res["graphs"],
+                   "softening": "single_spline", "mem": "host pinned" if res["pinned"] else "host pageable",
+                   "positions": "posstruct (AoS)" if res["aos"] else "PosXYZ", "fused_kick": res["fuse_kick"],
                    "planner": args.planner},
         "end_to_end": {
             "interactions_per_s": di / wall,
-            "median_step_ms": 1e3 * float(np.median(walls)),
-            "device_busy_s_per_step": busy / args.steps,
+            "median_step_ms": 1e3 * float(np.median(res["walls"])),
+            "device_busy_s_per_step": busy / steps,
             "direct_kernel_interactions_per_s": di / kern if kern > 0 else None,
-            "h2d_s_per_step": up / args.steps, "d2h_s_per_step": down / args.steps,
-            "h2d_GB_per_step": sum(x["bytes_to_device"] for x in stats) / 1e9 / args.steps,
-            "d2h_GB_per_step": sum(x["bytes_from_device"] for x in stats) / 1e9 / args.steps,
-            "slabs_uploaded_per_step": sum(x["slabs_uploaded"] for x in stats) / args.steps,
-            "slabs_reused_per_step": sum(x["slabs_reused"] for x in stats) / args.steps,
+            "h2d_s_per_step": up / steps, "d2h_s_per_step": down / steps,
+            "h2d_GB_per_step": sum(x["bytes_to_device"] for x in stats) / 1e9 / steps,
+            "d2h_GB_per_step": sum(x["bytes_from_device"] for x in stats) / 1e9 / steps,
+            "slabs_uploaded_per_step": sum(x["slabs_uploaded"] for x in stats) / steps,
+            "slabs_reused_per_step": sum(x["slabs_reused"] for x in stats) / steps,
             "hidden_transfer_fraction": hidden,
-            "plan_s": plan_s,
-            "device_plan_s_per_step": sum(x["plan_s"] for x in stats) / args.steps,
+            "plan_s_before_timing": res["plan_s"],
+            "device_plan_s_per_step": sum(x["plan_s"] for x in stats) / steps,
         },
         "autotune": [{k: e[k] for k in ("occupancy_quarters", "size_class", "sinks_per_lane", "uses")}
-                     for e in table],
+                     for e in res["table"]],
     }
-    if not args.no_cpu and not args.fuse_kick:
-        results = {"batched": {"acc_sample": {x: acc[x][:2000] for x in (sinks[0], sinks[len(sinks) // 2])}}}
-        line["cpu_baseline"] = cpu_baseline(box, R, soft_len, args.cpu_seconds, results, slabs=sinks)
-    print(json.dumps(line), flush=True)
 
 
 def calibrate_copy(mib):
@@ -595,6 +688,23 @@ def cpu_baseline(box, R, soft_len, budget_s, results, slabs=None):
     out["gpu_vs_cpu_rel_diff"] = {p: {"rms": float(np.sqrt(np.mean(np.concatenate(v) ** 2))),
                                       "max": float(np.concatenate(v).max())} for p, v in agree.items()}
     return out
+
+
+def error_check(box, R, soft_len, accs, frac, seed, slabs=None):
+    """Part of the CPU leg: the GPU's accelerations ({path: {slab: acc}})
+    against the FP64 direct-summation oracle on a seeded sample of the sinks
+    (a fraction ``frac`` of each slab's), as rms and max relative error over
+    the sample, outside softening-dominated sinks, and over those of condition
+    number <= 100 (oracle/pyoracle.py error_report)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    eps, _ = synth.eps_param("single_spline", soft_len)
+    b2 = synth.fof_b2(box.mean_spacing())
+    t0 = time.perf_counter()
+    rep = po.error_report(box, R, "single_spline", eps, b2, accs, frac, seed, slabs)
+    rep["oracle_s"] = time.perf_counter() - t0
+    log(f"[bench] FP64 check of {box.name}: {rep['sample']['sinks']} sinks in {rep['oracle_s']:.1f}s")
+    return rep
 
 
 if __name__ == "__main__":

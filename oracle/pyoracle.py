@@ -138,6 +138,41 @@ def sample_sinks(box, frac, seed, slabs=None):
     return np.concatenate(sl), np.concatenate(ix), np.concatenate(ce)
 
 
+def error_report(box, R, kind, eps, b2, accs, frac, seed, slabs=None):
+    """Relative acceleration errors |a - a_64| / |a_64| of each result in
+    ``accs`` ({name: {slab: (n, 4) float32 acc}}) on a seeded sample of sinks
+    (sample_sinks) against direct_f64: rms and max over the whole sample,
+    outside softening-dominated sinks (pairs closer than the softening length
+    carry more than half of |a_64|), and over the sinks outside those whose
+    condition number (sum of the pair terms' magnitudes over |a_64|) is at
+    most 100."""
+    sl, ix, ce = sample_sinks(box, frac, seed, slabs)
+    ref, soft, mag = direct_f64(box, R, kind, eps, b2, sl, ix, ce, with_soft=True, with_abs=True)
+    an = np.linalg.norm(ref[:, :3], axis=1)
+    keep = soft <= 0.5 * an
+    cond = mag / np.maximum(an, 1e-300)
+    out = {"sample": {"fraction": frac, "seed": seed, "sinks": int(sl.size),
+                      "softening_dominated": int((~keep).sum()),
+                      "condition_number_above_100": int((keep & (cond > 100)).sum())}}
+
+    def stats(rel, mask):
+        r = rel[mask]
+        if r.size == 0:
+            return None
+        return {"n": int(r.size), "rms": float(np.sqrt(np.mean(r ** 2))), "max": float(r.max()),
+                "above_1e-5": int((r > 1e-5).sum())}
+
+    for name, acc in accs.items():
+        got = np.empty((sl.size, 3))
+        for x in np.unique(sl):
+            m = sl == x
+            got[m] = acc[int(x)][ix[m], :3]
+        rel = np.linalg.norm(got - ref[:, :3], axis=1) / np.maximum(an, 1e-300)
+        out[name] = {"all": stats(rel, np.ones(rel.size, bool)), "outside_softening_dominated": stats(rel, keep),
+                     "condition_number_le_100": stats(rel, keep & (cond <= 100))}
+    return out
+
+
 def calc_di(box, R, x):
     lib = _load("liboracle.so")
     lib.or_calc_di.restype = C.c_uint64
