@@ -301,6 +301,66 @@ GPlanOut plan_out(char *base, const PlanLayout &L, int sic, int32_t *shared, int
     return o;
 }
 
+// A host-planned SIC's arrays into the plan image `hp` at the offsets of L,
+// with the two lists derived from them: the source pencil of each padded
+// 64-source block (-1: a block of no pencil, skipped) and the co-add chunks.
+void fill_plan_image(const nf_sic &s, const PlanLayout &L, char *hp) {
+    std::memcpy(hp + L.o_sink_start, s.SinkSetStart, 4 * (size_t)s.NSinkSets);
+    std::memcpy(hp + L.o_sink_count, s.SinkSetCount, 4 * (size_t)s.NSinkSets);
+    std::memcpy(hp + L.o_idmax, s.SinkSetIdMax, 4 * (size_t)s.NSinkSets);
+    std::memcpy(hp + L.o_sink_plan, s.SinkPlan, sizeof(nf_sink_plan) * (size_t)s.NSinkSets);
+    std::memcpy(hp + L.o_src_start, s.SourceSetStart, 4 * (size_t)s.NSourceSets);
+    std::memcpy(hp + L.o_src_count, s.SourceSetCount, 4 * (size_t)s.NSourceSets);
+    std::memcpy(hp + L.o_src_plan, s.SourcePlan, sizeof(nf_source_plan) * (size_t)s.NSourceSets);
+    if (s.NSinkBlocks) std::memcpy(hp + L.o_parent, s.SinkBlockParentPencil, 4 * (size_t)s.NSinkBlocks);
+    std::memcpy(hp + L.o_ilist, s.SinkSourceInteractionList, 4 * (size_t)s.InteractionCount);
+    std::memcpy(hp + L.o_yoff, s.SinkSourceYOffset, 4 * (size_t)s.InteractionCount);
+    int32_t *sp = reinterpret_cast<int32_t *>(hp + L.o_src_parent);
+    std::fill(sp, sp + s.NSourceBlocks, -1);
+    for (int k = 0; k < s.NSourceSets; k++) {
+        const int b0 = s.SourceSetStart[k] / NF_BLOCK_SIZE;
+        const int nb = (s.SourceSetCount[k] + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE;
+        for (int b = 0; b < nb; b++) sp[b0 + b] = k;
+    }
+    int32_t *ch = reinterpret_cast<int32_t *>(hp + L.o_chunks);
+    int64_t q = 0;
+    for (int k = 0; k < s.NSinkSets; k++)
+        for (int f = 0; f < s.SinkPlan[k].cell[0].N; f += NF_BLOCK_SIZE) {
+            ch[2 * q] = k;
+            ch[2 * q + 1] = f;
+            q++;
+        }
+}
+
+// The device views of a SIC's plan arrays at the offsets of L in `pd`.
+DevPlan dev_plan(const nf_sic &s, const PlanLayout &L, char *pd) {
+    DevPlan pl{};
+    pl.sink_set_start = reinterpret_cast<const int32_t *>(pd + L.o_sink_start);
+    pl.sink_set_count = reinterpret_cast<const int32_t *>(pd + L.o_sink_count);
+    pl.sink_idmax = reinterpret_cast<const uint32_t *>(pd + L.o_idmax);
+    pl.sink_plan = reinterpret_cast<const nf_sink_plan *>(pd + L.o_sink_plan);
+    pl.src_set_start = reinterpret_cast<const int32_t *>(pd + L.o_src_start);
+    pl.src_set_count = reinterpret_cast<const int32_t *>(pd + L.o_src_count);
+    pl.src_plan = reinterpret_cast<const nf_source_plan *>(pd + L.o_src_plan);
+    pl.parent = reinterpret_cast<const int32_t *>(pd + L.o_parent);
+    pl.ilist = reinterpret_cast<const int32_t *>(pd + L.o_ilist);
+    pl.yoff = reinterpret_cast<const float *>(pd + L.o_yoff);
+    pl.src_parent = reinterpret_cast<const int32_t *>(pd + L.o_src_parent);
+    pl.coadd_chunk = reinterpret_cast<const int32_t *>(pd + L.o_chunks);
+    pl.n_coadd_chunks = L.n_chunks;
+    pl.nsink_sets = s.NSinkSets;
+    pl.nsrc_sets = s.NSourceSets;
+    pl.nsink_blocks = s.NSinkBlocks;
+    pl.nsrc_blocks = s.NSourceBlocks;
+    pl.W = s.nfwidth;
+    pl.R = s.nfradius;
+    pl.cpd = s.cpd;
+    pl.Nk = s.Nk;
+    pl.j_width = s.j_width;
+    pl.j_low = s.j_low;
+    return pl;
+}
+
 // The scalar members of a SIC of slab `slab`, rows [j_low, j_high), as the
 // SetInteractionCollection constructor sets them (PencilOnPencil.cc:180-215)
 // for the non-PARALLEL layout.
@@ -447,8 +507,9 @@ uint64_t tile_split_cost(int softening, int S, uint64_t interactions) {
                            (uint64_t)std::max(1, batched_kernel_waves(softening, S));
     return std::max<uint64_t>(kSplitMinCost, interactions / waves / 4);
 }
-void build_items(const nf_sic &s, int S, int sic, uint64_t max_tile_cost, std::vector<WorkItem> &items,
-                 std::vector<uint16_t> &bucket) {
+// Calls emit(item, bucket) for each work item of the SIC, in list order.
+template <typename Emit>
+void for_each_item(const nf_sic &s, int S, int sic, uint64_t max_tile_cost, Emit &&emit) {
     const int tile = NF_BLOCK_SIZE * S;
     const int W = s.nfwidth;
     for (int z = 0; z < s.Nk; z++)
@@ -461,16 +522,10 @@ void build_items(const nf_sic &s, int S, int sic, uint64_t max_tile_cost, std::v
             int f = 0;
             if (S > 1 && (uint64_t)tile * nsrc > max_tile_cost) {
                 const uint16_t b = cost_bucket((uint64_t)NF_BLOCK_SIZE * nsrc);
-                for (; f + NF_BLOCK_SIZE <= n; f += NF_BLOCK_SIZE) {
-                    items.push_back(WorkItem{k, f, NF_BLOCK_SIZE, 2 + 4 * sic});
-                    bucket.push_back(b);
-                }
+                for (; f + NF_BLOCK_SIZE <= n; f += NF_BLOCK_SIZE) emit(WorkItem{k, f, NF_BLOCK_SIZE, 2 + 4 * sic}, b);
             } else if (n >= tile) {
                 const uint16_t b = cost_bucket((uint64_t)tile * nsrc);
-                for (; f + tile <= n; f += tile) {
-                    items.push_back(WorkItem{k, f, tile, 0 + 4 * sic});
-                    bucket.push_back(b);
-                }
+                for (; f + tile <= n; f += tile) emit(WorkItem{k, f, tile, 0 + 4 * sic}, b);
             }
             // The tail: one-wave tiles while they keep at least kTailLanes
             // lanes busy, then the rest with sources on the lanes.
@@ -478,16 +533,20 @@ void build_items(const nf_sic &s, int S, int sic, uint64_t max_tile_cost, std::v
                 const uint16_t b = cost_bucket((uint64_t)NF_BLOCK_SIZE * nsrc);
                 while (n - f >= kTailLanes) {
                     const int m = std::min(NF_BLOCK_SIZE, n - f);
-                    items.push_back(WorkItem{k, f, m, 2 + 4 * sic});
-                    bucket.push_back(b);
+                    emit(WorkItem{k, f, m, 2 + 4 * sic}, b);
                     f += m;
                 }
             }
-            if (f < n) {
-                items.push_back(WorkItem{k, f, n - f, 1 + 4 * sic});
-                bucket.push_back(cost_bucket((uint64_t)(n - f) * nsrc));
-            }
+            if (f < n) emit(WorkItem{k, f, n - f, 1 + 4 * sic}, cost_bucket((uint64_t)(n - f) * nsrc));
         }
+}
+
+void build_items(const nf_sic &s, int S, int sic, uint64_t max_tile_cost, std::vector<WorkItem> &items,
+                 std::vector<uint16_t> &bucket) {
+    for_each_item(s, S, sic, max_tile_cost, [&](const WorkItem &w, uint16_t b) {
+        items.push_back(w);
+        bucket.push_back(b);
+    });
 }
 
 int validate(const nf_sic &s) {
@@ -1031,60 +1090,10 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     for (int i = 0; i < n; i++) {
         const nf_sic &s = g[i]->sic;
         const PlanLayout &L = PL[i];
-        if (!g[i]->dp.on) {  // host-planned: copy the arrays, derive the lists
-            std::memcpy(hp + L.o_sink_start, s.SinkSetStart, 4 * (size_t)s.NSinkSets);
-            std::memcpy(hp + L.o_sink_count, s.SinkSetCount, 4 * (size_t)s.NSinkSets);
-            std::memcpy(hp + L.o_idmax, s.SinkSetIdMax, 4 * (size_t)s.NSinkSets);
-            std::memcpy(hp + L.o_sink_plan, s.SinkPlan, sizeof(nf_sink_plan) * (size_t)s.NSinkSets);
-            std::memcpy(hp + L.o_src_start, s.SourceSetStart, 4 * (size_t)s.NSourceSets);
-            std::memcpy(hp + L.o_src_count, s.SourceSetCount, 4 * (size_t)s.NSourceSets);
-            std::memcpy(hp + L.o_src_plan, s.SourcePlan, sizeof(nf_source_plan) * (size_t)s.NSourceSets);
-            if (s.NSinkBlocks) std::memcpy(hp + L.o_parent, s.SinkBlockParentPencil, 4 * (size_t)s.NSinkBlocks);
-            std::memcpy(hp + L.o_ilist, s.SinkSourceInteractionList, 4 * (size_t)s.InteractionCount);
-            std::memcpy(hp + L.o_yoff, s.SinkSourceYOffset, 4 * (size_t)s.InteractionCount);
-            {   // block -> source pencil, and the co-add chunks
-                int32_t *sp = reinterpret_cast<int32_t *>(hp + L.o_src_parent);
-                std::fill(sp, sp + s.NSourceBlocks, -1);  // blocks of no pencil are skipped
-                for (int k = 0; k < s.NSourceSets; k++) {
-                    const int b0 = s.SourceSetStart[k] / NF_BLOCK_SIZE;
-                    const int nb = (s.SourceSetCount[k] + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE;
-                    for (int b = 0; b < nb; b++) sp[b0 + b] = k;
-                }
-                int32_t *ch = reinterpret_cast<int32_t *>(hp + L.o_chunks);
-                int64_t q = 0;
-                for (int k = 0; k < s.NSinkSets; k++)
-                    for (int f = 0; f < s.SinkPlan[k].cell[0].N; f += NF_BLOCK_SIZE) {
-                        ch[2 * q] = k;
-                        ch[2 * q + 1] = f;
-                        q++;
-                    }
-            }
-        }
+        if (!g[i]->dp.on) fill_plan_image(s, L, hp);  // host-planned: copy the arrays, derive the lists
 
         DevPlan &pl = plans[i];
-        pl.sink_set_start = reinterpret_cast<const int32_t *>(pd + L.o_sink_start);
-        pl.sink_set_count = reinterpret_cast<const int32_t *>(pd + L.o_sink_count);
-        pl.sink_idmax = reinterpret_cast<const uint32_t *>(pd + L.o_idmax);
-        pl.sink_plan = reinterpret_cast<const nf_sink_plan *>(pd + L.o_sink_plan);
-        pl.src_set_start = reinterpret_cast<const int32_t *>(pd + L.o_src_start);
-        pl.src_set_count = reinterpret_cast<const int32_t *>(pd + L.o_src_count);
-        pl.src_plan = reinterpret_cast<const nf_source_plan *>(pd + L.o_src_plan);
-        pl.parent = reinterpret_cast<const int32_t *>(pd + L.o_parent);
-        pl.ilist = reinterpret_cast<const int32_t *>(pd + L.o_ilist);
-        pl.yoff = reinterpret_cast<const float *>(pd + L.o_yoff);
-        pl.src_parent = reinterpret_cast<const int32_t *>(pd + L.o_src_parent);
-        pl.coadd_chunk = reinterpret_cast<const int32_t *>(pd + L.o_chunks);
-        pl.n_coadd_chunks = L.n_chunks;
-        pl.nsink_sets = s.NSinkSets;
-        pl.nsrc_sets = s.NSourceSets;
-        pl.nsink_blocks = s.NSinkBlocks;
-        pl.nsrc_blocks = s.NSourceBlocks;
-        pl.W = s.nfwidth;
-        pl.R = s.nfradius;
-        pl.cpd = s.cpd;
-        pl.Nk = s.Nk;
-        pl.j_width = s.j_width;
-        pl.j_low = s.j_low;
+        pl = dev_plan(s, L, pd);
 
         DevSlabs &sl = slabs[i];
         for (int c = 0; c < s.nfwidth; c++) {
@@ -1985,5 +1994,168 @@ extern "C" int nf_stream_end(void) {
     g_rt->cache.active = false;
     for (auto &sp : g_rt->cache.slots)
         if (sp->refs == 0) sp->valid = false;  // slabs in use are dropped on release
+    return NF_OK;
+}
+
+// ---- The entry point that allocates nothing ----------------------------
+
+namespace nf {
+namespace {
+
+// Device scratch of nf_execute_sic_noalloc: a SIC's plan image (view, work
+// list, queue counter, plan arrays), its source stream with the block boxes,
+// and its FP64 partials.  Each GPU holds its own copy from the library's
+// first use on it until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device memory: kernels, copies and fills
+// take the address hipGetSymbolAddress gives for the current device.
+__device__ __align__(256) char scratch[NF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kNoallocMaxDevices = 64;
+char *g_noalloc_dev[kNoallocMaxDevices];  // per device: the address of `scratch` there
+
+// Host image of the upload (view, work list, plan arrays) and the counting
+// sort's cursors, static so that a call allocates no host memory either.
+constexpr size_t kNoallocHostBytes = 32u << 20;
+alignas(256) char g_noalloc_host[kNoallocHostBytes];
+uint32_t g_noalloc_cursor[kItemBuckets + 1];
+
+// Calls share the scratch of their GPU and the host image: one at a time.
+std::mutex g_noalloc_m;
+
+// Where one SIC sits in the scratch: [view][items][counter][plan arrays]
+// (the uploaded image) then [source stream + boxes][partials].
+struct NoallocLayout {
+    size_t o_items, o_counter, o_plan, image, o_work, total;
+    size_t items_max;
+    PlanLayout plan;
+    WorkLayout work;
+};
+
+NoallocLayout noalloc_layout(const nf_sic &s) {
+    NoallocLayout L{};
+    // At most one item per padded sink block plus one tail per pencil,
+    // whatever the tile split.
+    L.items_max = (size_t)s.NSinkBlocks + (size_t)s.NSinkSets;
+    L.o_items = align_up(sizeof(SicView), 256);
+    L.o_counter = align_up(L.o_items + sizeof(WorkItem) * std::max<size_t>(L.items_max, 1), 256);
+    L.o_plan = L.o_counter + 256;
+    L.plan = plan_layout(s, L.o_plan, count_coadd_chunks(s));
+    L.image = L.o_plan + L.plan.total;
+    L.o_work = align_up(L.image, 256);
+    L.work = work_layout(s, NF_PATH_BATCHED, L.o_work);
+    L.total = L.o_work + L.work.total;
+    return L;
+}
+
+// The address of `scratch` on device `dev` (current), looked up once.
+int noalloc_scratch(int dev, char **out) {
+    if (dev < 0 || dev >= kNoallocMaxDevices) return fail(NF_ERR_NODEV, "device index beyond the scratch table");
+    if (!g_noalloc_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(NF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noalloc_dev[dev] = static_cast<char *>(p);
+    }
+    *out = g_noalloc_dev[dev];
+    return NF_OK;
+}
+
+}  // namespace
+}  // namespace nf
+
+extern "C" int nf_execute_sic_noalloc(const nf_sic *sic, void *stream) {
+    if (!sic) return fail(NF_ERR_INVALID, "null argument");
+    const nf_sic &s = *sic;
+    if (s.mem != NF_MEM_DEVICE)
+        return fail(NF_ERR_INVALID, "nf_execute_sic_noalloc takes device-resident slabs (mem = NF_MEM_DEVICE)");
+    int rc = validate(s);
+    if (rc) return rc;
+    PairConst pc;
+    if ((rc = derive_pair_const(s.softening, s.eps, s.b2, &pc))) return rc;
+    const NoallocLayout L = noalloc_layout(s);
+    if (L.total > NF_NOALLOC_SCRATCH_BYTES)
+        return fail(NF_ERR_CAPACITY, "SIC needs " + std::to_string(L.total) + " bytes of device scratch; "
+                                     "nf_execute_sic_noalloc holds " + std::to_string((uint64_t)NF_NOALLOC_SCRATCH_BYTES) +
+                                     " (split the slab's rows into smaller SICs)");
+    if (L.image > kNoallocHostBytes)
+        return fail(NF_ERR_CAPACITY, "SIC plan image of " + std::to_string(L.image) + " bytes exceeds the " +
+                                         std::to_string((uint64_t)kNoallocHostBytes) + "-byte host image");
+
+    std::lock_guard<std::mutex> lk(g_noalloc_m);
+    int dev = 0;
+    NF_HIP(hipGetDevice(&dev));
+    char *d = nullptr;
+    if ((rc = noalloc_scratch(dev, &d))) return rc;
+    char *h = g_noalloc_host;
+
+    // Plan arrays and the lists derived from them.
+    fill_plan_image(s, L.plan, h);
+    const DevPlan pl = dev_plan(s, L.plan, d);
+
+    // The work list, longest items first (items within 1/8 octave of cost
+    // keep sink-pencil order): a counting sort by cost bucket in two passes.
+    constexpr int S = 2;  // every tile shape gives bitwise the same partials
+    const uint64_t split = tile_split_cost(s.softening, S, s.DirectTotal);
+    std::fill(g_noalloc_cursor, g_noalloc_cursor + kItemBuckets + 1, 0u);
+    size_t nitems = 0;
+    for_each_item(s, S, 0, split, [&](const WorkItem &, uint16_t b) {
+        g_noalloc_cursor[b + 1]++;
+        nitems++;
+    });
+    if (nitems > L.items_max) return fail(NF_ERR_STATE, "work list longer than its bound");
+    for (int b = 0; b < kItemBuckets; b++) g_noalloc_cursor[b + 1] += g_noalloc_cursor[b];
+    WorkItem *items = reinterpret_cast<WorkItem *>(h + L.o_items);
+    for_each_item(s, S, 0, split, [&](const WorkItem &w, uint16_t b) { items[g_noalloc_cursor[b]++] = w; });
+
+    DevSlabs sl{};
+    for (int c = 0; c < s.nfwidth; c++) {
+        sl.n[c] = s.Nslab[c];
+        sl.pos[c] = s.SourcePosSlab[c];
+    }
+    sl.pos[s.nfradius] = s.SinkPosSlab;
+    sl.aos = s.pos_layout == NF_POS_AOS ? 1 : 0;
+    KickArgs ka;
+    std::memset(&ka, 0, sizeof(ka));
+    if (s.kick) {
+        ka.on = 1;
+        ka.rescale = (float)s.kick->rescale;
+        ka.inv_eps3 = (float)s.kick->inv_eps3;
+        ka.density_rad2 = s.kick->density_rad2;
+        ka.far = s.kick->FarAccSlab;
+        ka.near_out = s.kick->NearAccOut;
+        ka.far_base = 0;
+    }
+    char *src = d + L.work.o_src;
+    SicView &v = *reinterpret_cast<SicView *>(h);
+    std::memset(&v, 0, sizeof(v));
+    v.sink_set_start = pl.sink_set_start;
+    v.src_set_start = pl.src_set_start;
+    v.src_set_count = pl.src_set_count;
+    v.ilist = pl.ilist;
+    v.sink_plan = pl.sink_plan;
+    v.sink_slab = s.SinkPosSlab;
+    v.sink_n = s.Nslab[s.nfradius];
+    v.aos = sl.aos;
+    v.sources4 = src;
+    v.box4 = box_after(src, s.NSourceBlocks);
+    v.partial = reinterpret_cast<double *>(d + L.work.o_part);
+    v.W = s.nfwidth;
+    v.R = s.nfradius;
+
+    // Upload, gather, one persistent launch, co-add, on the caller's stream;
+    // the queue counter is cleared by a kernel inside launch_batched.  The
+    // host image stays untouched until the stream has drained.
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    NF_HIP(hipMemcpyAsync(d, h, L.image, hipMemcpyHostToDevice, st));
+    int e = launch_gather(pl, sl, true, nullptr, src, st);
+    if (!e)
+        e = launch_batched(reinterpret_cast<const SicView *>(d), s.cpd, s.softening, pc, TileShape{S, 64, 4, 0},
+                           reinterpret_cast<const WorkItem *>(d + L.o_items), (int32_t)nitems,
+                           reinterpret_cast<int32_t *>(d + L.o_counter), st);
+    if (!e) e = launch_coadd_f64(pl, d + L.work.o_part, pc.out_scale, s.SinkAccSlab, ka, st);
+    const hipError_t se = hipStreamSynchronize(st);
+    if (e) return fail(NF_ERR_HIP, std::string("nf_execute_sic_noalloc launch: ") + hipGetErrorString((hipError_t)e));
+    if (se != hipSuccess) return fail(NF_ERR_HIP, std::string("nf_execute_sic_noalloc: ") + hipGetErrorString(se));
     return NF_OK;
 }

@@ -147,6 +147,7 @@ def load_library(path=LIB_PATH):
     lib.nf_get_throughput.argtypes = [C.POINTER(nf_throughput)]
     lib.nf_stream_begin.argtypes = []
     lib.nf_stream_end.argtypes = []
+    lib.nf_execute_sic_noalloc.argtypes = [C.POINTER(nf_sic), C.c_void_p]
     return lib
 
 
@@ -245,6 +246,17 @@ class PinnedArray:
 
 def device_synchronize():
     check(lib().nf_device_synchronize(), "nf_device_synchronize")
+
+
+# Device scratch of nf_execute_sic_noalloc (NF_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 256 << 20
+
+
+def execute_sic_noalloc(sic, stream=None):
+    """nf_execute_sic_noalloc: run one planned, device-resident SIC on the
+    batched path with the module's fixed device scratch (no allocation, no
+    nf_setup needed); returns when its accelerations are in place."""
+    check(lib().nf_execute_sic_noalloc(C.byref(sic.desc), stream), "nf_execute_sic_noalloc")
 
 
 class SetInteractionCollection:

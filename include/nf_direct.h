@@ -374,6 +374,24 @@ int nf_plan_sic_device(int32_t cpd, int32_t nfradius, int32_t slab, int32_t j_lo
                        const uint64_t *nslab, float eps, float b2, int32_t softening, void *arena,
                        uint64_t arena_bytes, nf_sic *out);
 
+/* ---- execution without allocation -------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nf_submit(sic, blocking = 1) on the batched path, for a
+ * SIC whose slabs are device-resident (mem = NF_MEM_DEVICE), with every
+ * scratch buffer (plan image, work list, queue counter, source stream,
+ * partials) in one fixed device array of NF_NOALLOC_SCRATCH_BYTES per GPU,
+ * held from the first call on that GPU until the process ends.  It needs no
+ * nf_setup, allocates no memory (device or host) when it succeeds, runs on
+ * `stream` (a hipStream_t; NULL: the null stream) of the current device and
+ * returns when the SIC's accelerations are in sic->SinkAccSlab: bitwise those
+ * nf_submit gives on the batched path.  A SIC that needs more scratch fails with
+ * NF_ERR_CAPACITY (split the slab's rows into smaller SICs, as
+ * nf_split_slab does for nf_setup's buffers).  Calls share their GPU's array
+ * and are serialized by a lock; the array keeps the last call's contents. */
+#define NF_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int nf_execute_sic_noalloc(const nf_sic *sic, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

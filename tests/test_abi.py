@@ -117,3 +117,29 @@ def test_runtime_errors_without_setup():
 def test_no_cpu_fallback_when_library_missing(tmp_path):
     with pytest.raises(nfm.NFError):
         nfm.load_library(str(tmp_path / "libnfdirect.so"))
+
+
+def test_noalloc_rejects_before_touching_a_gpu():
+    """nf_execute_sic_noalloc checks its SIC before any HIP call: a null or
+    host-resident SIC is invalid, and one whose working set exceeds the fixed
+    scratch fails with NF_ERR_CAPACITY naming both sizes."""
+    import synth
+    lib = nfm.lib()
+    assert lib.nf_execute_sic_noalloc(None, None) == -1
+    box = small_uniform(cpd=7, ppc=4, seed=1)
+    eps, b2, _ = params(box)
+    sic = nfm.SetInteractionCollection(box, 3, 0, 7, R, eps, b2, "single_spline", mem=nfm.NF_MEM_HOST)
+    assert lib.nf_execute_sic_noalloc(C.byref(sic.desc), None) == -1
+    assert b"NF_MEM_DEVICE" in lib.nf_last_error()
+    # 30,000 particles in every cell of a CPD 7 box: about 2.5 times the
+    # scratch.  The positions are never read (lazily zeroed pages).
+    cpd, per_cell = 7, 30000
+    slabs = [synth.Slab(np.zeros(3 * cpd * cpd * per_cell, np.float32), np.full(cpd * cpd, per_cell, np.int32))
+             for _ in range(cpd)]
+    heavy = synth.Box(cpd, slabs, "heavy")
+    fake = [0x1000 * (c + 1) for c in range(2 * R + 1)]  # device addresses the check never follows
+    sic = nfm.SetInteractionCollection(heavy, 3, 0, 7, R, eps, b2, "single_spline", pos_ptrs=fake,
+                                       acc_ptr=0x100000, mem=nfm.NF_MEM_DEVICE)
+    assert lib.nf_execute_sic_noalloc(C.byref(sic.desc), None) == -3
+    msg = lib.nf_last_error().decode()
+    assert "scratch" in msg and str(nfm.NOALLOC_SCRATCH_BYTES) in msg
