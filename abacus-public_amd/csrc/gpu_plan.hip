@@ -47,6 +47,12 @@ __device__ __forceinline__ void wave_add(uint64_t *p, uint64_t v) {
     if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long *)p, (unsigned long long)v);
 }
 
+// The same for a 32-bit counter.
+__device__ __forceinline__ void wave_add32(int32_t *p, int32_t v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(p, v);
+}
+
 // Whether a planned cell (start as the plan stores it) lies inside its slab.
 __device__ __forceinline__ bool cell_ok(uint64_t start, int32_t n, uint64_t nslab) {
     return n >= 0 && (uint64_t)(uint32_t)start + (uint64_t)n <= nslab;
@@ -144,10 +150,13 @@ __global__ void k_plan_zero(GPlanTotals *tot, int32_t *shared, int nshared) {
     for (int i = threadIdx.x; i < nshared; i += blockDim.x) shared[i] = 0;
 }
 
-// Threads [0, P): sink pencils; [P, P+Q): source pencils.
-__global__ __launch_bounds__(kT) void k_plan_a(GPlanIn in, GPlanTotals *tot) {
+// Threads [0, P): sink pencils; [P, P+Q): source pencils; bid is the
+// workgroup's index within the SIC's grid.  group (or null): the launch
+// group's shared area, which receives this SIC's work items per tile and its
+// bad cells.
+__device__ void plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *group, int bid) {
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = bid * kT + threadIdx.x;
     uint64_t sink = 0, sblk = 0, ch = 0, it[3] = {0, 0, 0}, occ[4] = {0, 0, 0, 0}, src = 0, qblk = 0, bad = 0,
              d = 0, pd = 0;
     if (t < P) {
@@ -190,22 +199,30 @@ __global__ __launch_bounds__(kT) void k_plan_a(GPlanIn in, GPlanTotals *tot) {
     wave_add(&tot->bad, bad);
     wave_add(&tot->direct_total, d);
     wave_add(&tot->padded_direct_total, pd);
+    if (group) {
+        for (int i = 0; i < 3; i++) wave_add32(group + kPlanGroupItems + i, (int32_t)it[i]);
+        wave_add32(group + kPlanGroupBad, bad ? 1 : 0);  // pencils with a bad cell
+    }
+}
+__global__ __launch_bounds__(kT) void k_plan_a(GPlanIn in, GPlanTotals *tot, int32_t *group) {
+    plan_a(in, tot, group, blockIdx.x);
 }
 
 // ---- Phase B ---------------------------------------------------------------
 
 // Workgroups [0, GP): sink pencils kT at a time; [GP, GP+GQ): source pencils.
-__global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) {
+__device__ void plan_b1(const GPlanIn &in, const GPlanOut &out, int bid) {
+    if (out.group && out.group[kPlanGroupBad]) return;  // a cell outside its slab: plan nothing
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
     const int GP = ngroups(P);
-    if ((int)blockIdx.x < GP) {
+    if (bid < GP) {
         // The work-list histogram is built in LDS, then added to the group's
         // with one atomic per workgroup and bucket: nearly every pencil of a
         // uniform slab falls in the same bucket.
         __shared__ int32_t lh[3 * kPlanBuckets];
         for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT) lh[i] = 0;
         __syncthreads();
-        const int s = blockIdx.x * kT + threadIdx.x;
+        const int s = bid * kT + threadIdx.x;
         int nb = 0, nc = 0;
         if (s < P) {
             const int j = s / in.cpd, k = s % in.cpd;
@@ -247,13 +264,13 @@ __global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) {
         (void)block_excl_scan(nb, &tb);  // (its barriers also order the LDS histogram)
         (void)block_excl_scan(nc, &tc);
         if (threadIdx.x == 0) {
-            out.part[blockIdx.x] = tb;
-            out.part[GP + blockIdx.x] = tc;
+            out.part[bid] = tb;
+            out.part[GP + bid] = tc;
         }
         for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT)
             if (lh[i]) atomicAdd(out.hist + i, lh[i]);
     } else {
-        const int g = blockIdx.x - GP;
+        const int g = bid - GP;
         const int u = g * kT + threadIdx.x;
         int nb = 0;
         if (u < Q) {
@@ -280,6 +297,7 @@ __global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) {
         if (threadIdx.x == 0) out.part[2 * GP + g] = tb;
     }
 }
+__global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) { plan_b1(in, out, blockIdx.x); }
 
 // The work items of the workgroup's pencils, taken in the host list's order
 // (thread t: pencil (j, k) = (t % jw, t / jw), so neighbours in y -- which
@@ -288,12 +306,12 @@ __global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) {
 // workgroup's share of its bucket (LDS atomics), then the share's base (one
 // global atomic per workgroup and bucket).  boff: the buckets' offsets in
 // each list.  Every thread of the workgroup must call it.
-__device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_t (*boff)[kPlanBuckets]) {
+__device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_t (*boff)[kPlanBuckets], int bid) {
     __shared__ int32_t lc[3 * kPlanBuckets];
     const int P = in.jw * in.cpd;
     for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT) lc[i] = 0;
     __syncthreads();
-    const int t = blockIdx.x * kT + threadIdx.x;
+    const int t = bid * kT + threadIdx.x;
     const int s = t < P ? (t % in.jw) * in.cpd + t / in.jw : 0;
     const int n = t < P ? out.sink_count[s] : 0;
     const uint64_t nsrc = n > 0 ? sink_pencil_sources(in, s / in.cpd, s % in.cpd) : 0;
@@ -334,10 +352,11 @@ __device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_
 }
 
 // Same workgroup layout as k_plan_b1.
-__global__ __launch_bounds__(kT) void k_plan_b2(GPlanIn in, GPlanOut out) {
+__device__ void plan_b2(const GPlanIn &in, const GPlanOut &out, int bid) {
+    if (out.group && out.group[kPlanGroupBad]) return;
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
     const int GP = ngroups(P);
-    if ((int)blockIdx.x < GP) {
+    if (bid < GP) {
         // Bucket offsets of the wanted work lists: exclusive scan of the
         // histogram in LDS (2 buckets per thread).
         __shared__ int32_t boff[3][kPlanBuckets];
@@ -350,13 +369,13 @@ __global__ __launch_bounds__(kT) void k_plan_b2(GPlanIn in, GPlanOut out) {
             boff[ti][2 * threadIdx.x] = e;
             boff[ti][2 * threadIdx.x + 1] = e + a;
         }
-        const int s = blockIdx.x * kT + threadIdx.x;
+        const int s = bid * kT + threadIdx.x;
         const int n = s < P ? out.sink_count[s] : 0;
         const int n0 = s < P ? out.sink_plan[s].cell[0].N : 0;
         int tb, tc;
-        const int b0 = block_sum_prefix(out.part, blockIdx.x) + block_excl_scan(nblocks(n), &tb);
-        const int c0 = block_sum_prefix(out.part + GP, blockIdx.x) + block_excl_scan(nblocks(n0), &tc);
-        place_items(in, out, boff);
+        const int b0 = block_sum_prefix(out.part, bid) + block_excl_scan(nblocks(n), &tb);
+        const int c0 = block_sum_prefix(out.part + GP, bid) + block_excl_scan(nblocks(n0), &tc);
+        place_items(in, out, boff, bid);
         if (s >= P) return;
         const int j = s / in.cpd, k = s % in.cpd;
         out.sink_start[s] = b0 * kB;
@@ -372,7 +391,7 @@ __global__ __launch_bounds__(kT) void k_plan_b2(GPlanIn in, GPlanOut out) {
             out.chunks[2 * q + 1] = f;
         }
     } else {
-        const int g = blockIdx.x - GP;
+        const int g = bid - GP;
         const int u = g * kT + threadIdx.x;
         const int n = u < Q ? out.src_count[u] : 0;
         int tb;
@@ -382,6 +401,47 @@ __global__ __launch_bounds__(kT) void k_plan_b2(GPlanIn in, GPlanOut out) {
         for (int q = 0, nb = nblocks(n); q < nb; q++) out.src_parent[b0 + q] = u;
     }
 }
+__global__ __launch_bounds__(kT) void k_plan_b2(GPlanIn in, GPlanOut out) { plan_b2(in, out, blockIdx.x); }
+
+// ---- A launch group in one launch per phase --------------------------------
+// The SICs' inputs and outputs are arrays in device memory; first[s] is SIC
+// s's first workgroup in the phase's grid (first[n]: the grid size).  Each
+// workgroup finds its SIC by bisection and runs that SIC's workgroup
+// bid = blockIdx.x - first[s], exactly as the one-SIC launch would.
+
+__device__ __forceinline__ int group_sic(const int32_t *first, int n, int b) {
+    int lo = 0, hi = n - 1;  // the last s with first[s] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Zero the group's n totals (256 bytes apart) and its shared area.
+__global__ void k_plan_zero_group(char *tot, int n, int32_t *shared, int nshared) {
+    constexpr int m = sizeof(GPlanTotals) / sizeof(uint64_t);
+    for (int i = threadIdx.x; i < n * m; i += blockDim.x)
+        reinterpret_cast<uint64_t *>(tot + 256 * (size_t)(i / m))[i % m] = 0;
+    for (int i = threadIdx.x; i < nshared; i += blockDim.x) shared[i] = 0;
+}
+
+__global__ __launch_bounds__(kT) void k_plan_a_group(const GPlanIn *__restrict__ ins, char *tot, int32_t *group,
+                                                     const int32_t *__restrict__ first, int n) {
+    const int s = group_sic(first, n, (int)blockIdx.x);
+    plan_a(ins[s], reinterpret_cast<GPlanTotals *>(tot + 256 * (size_t)s), group, (int)blockIdx.x - first[s]);
+}
+__global__ __launch_bounds__(kT) void k_plan_b1_group(const GPlanIn *__restrict__ ins, const GPlanOut *__restrict__ outs,
+                                                      const int32_t *__restrict__ first, int n) {
+    const int s = group_sic(first, n, (int)blockIdx.x);
+    plan_b1(ins[s], outs[s], (int)blockIdx.x - first[s]);
+}
+__global__ __launch_bounds__(kT) void k_plan_b2_group(const GPlanIn *__restrict__ ins, const GPlanOut *__restrict__ outs,
+                                                      const int32_t *__restrict__ first, int n) {
+    const int s = group_sic(first, n, (int)blockIdx.x);
+    plan_b2(ins[s], outs[s], (int)blockIdx.x - first[s]);
+}
 
 }  // namespace
 
@@ -389,11 +449,11 @@ size_t gpu_plan_part_ints(int nsink_sets, int nsrc_sets) {
     return 2 * (size_t)ngroups(nsink_sets) + (size_t)ngroups(nsrc_sets);
 }
 
-int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *shared, void *stream) {
+int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *zero, int32_t *group, void *stream) {
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
-    hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(kT), 0, st, tot, shared, shared ? kPlanSharedInts : 0);
-    hipLaunchKernelGGL(k_plan_a, dim3(ngroups(P + Q)), dim3(kT), 0, st, in, tot);
+    hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(kT), 0, st, tot, zero, zero ? kPlanSharedInts : 0);
+    hipLaunchKernelGGL(k_plan_a, dim3(ngroups(P + Q)), dim3(kT), 0, st, in, tot, group);
     return (int)hipGetLastError();
 }
 
@@ -408,6 +468,26 @@ int launch_gpu_plan_b2(const GPlanIn &in, const GPlanOut &out, void *stream) {
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
     hipLaunchKernelGGL(k_plan_b2, dim3(ngroups(P) + ngroups(Q)), dim3(kT), 0, static_cast<hipStream_t>(stream), in,
                        out);
+    return (int)hipGetLastError();
+}
+
+int gpu_plan_grid_a(const GPlanIn &in) { return ngroups((2 * in.jw + in.W - 1) * in.cpd); }
+int gpu_plan_grid_b(const GPlanIn &in) { return ngroups(in.jw * in.cpd) + ngroups((in.jw + in.W - 1) * in.cpd); }
+
+int launch_gpu_plan_group(int phase, const GPlanIn *ins, const GPlanOut *outs, GPlanTotals *tot, int32_t *shared,
+                          const int32_t *first, int n, int grid, void *stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (n <= 0 || grid <= 0) return 0;
+    char *t = reinterpret_cast<char *>(tot);
+    switch (phase) {
+        case 0:
+            hipLaunchKernelGGL(k_plan_zero_group, dim3(1), dim3(kT), 0, st, t, n, shared, kPlanSharedInts);
+            hipLaunchKernelGGL(k_plan_a_group, dim3(grid), dim3(kT), 0, st, ins, t, shared, first, n);
+            break;
+        case 1: hipLaunchKernelGGL(k_plan_b1_group, dim3(grid), dim3(kT), 0, st, ins, outs, first, n); break;
+        case 2: hipLaunchKernelGGL(k_plan_b2_group, dim3(grid), dim3(kT), 0, st, ins, outs, first, n); break;
+        default: return (int)hipErrorInvalidValue;
+    }
     return (int)hipGetLastError();
 }
 

@@ -50,6 +50,20 @@ __device__ __forceinline__ const NF_CONST T *cptr(const void *p) {
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 
+// A SIC's counts: from device memory when it was planned on the device
+// without a host round trip (its host-side fields are then bounds), else
+// the host's.  plan_bad: its launch group holds a cell outside its slab.
+__device__ __forceinline__ bool plan_bad(const DevPlan &pl) { return pl.dbad && *cptr<int32_t>(pl.dbad) != 0; }
+__device__ __forceinline__ int sink_blocks(const DevPlan &pl) {
+    return pl.dtot ? (int)cptr<GPlanTotals>(pl.dtot)->sink_blocks : pl.nsink_blocks;
+}
+__device__ __forceinline__ int src_blocks(const DevPlan &pl) {
+    return pl.dtot ? (int)cptr<GPlanTotals>(pl.dtot)->source_blocks : pl.nsrc_blocks;
+}
+__device__ __forceinline__ int coadd_chunks(const DevPlan &pl) {
+    return pl.dtot ? (int)cptr<GPlanTotals>(pl.dtot)->chunks : pl.n_coadd_chunks;
+}
+
 // ---------------------------------------------------------------------------
 // Gathers
 // ---------------------------------------------------------------------------
@@ -79,12 +93,13 @@ constexpr int kGatherBlocks = 4;
 __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *__restrict__ slab, uint64_t n,
                                                       int aos, float4 *__restrict__ out, float cellsize) {
     const int b0 = (blockIdx.x * 4 + (int)(threadIdx.x / kWave)) * kGatherBlocks;
-    if (b0 >= pl.nsink_blocks) return;
+    const int nsb = sink_blocks(pl);
+    if (b0 >= nsb || plan_bad(pl)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int W = pl.W;
     int par[kGatherBlocks];
 #pragma unroll
-    for (int u = 0; u < kGatherBlocks; u++) par[u] = pl.parent[min(b0 + u, pl.nsink_blocks - 1)];
+    for (int u = 0; u < kGatherBlocks; u++) par[u] = pl.parent[min(b0 + u, nsb - 1)];
     uint64_t idx[kGatherBlocks];
     int dcell[kGatherBlocks];
     bool live[kGatherBlocks];
@@ -96,7 +111,7 @@ __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *_
         pre[0] = 0;
         for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
         const int i = (b0 + u) * kWave - pl.sink_set_start[s] + lane;  // index within the pencil
-        live[u] = b0 + u < pl.nsink_blocks && i < pre[W];
+        live[u] = b0 + u < nsb && i < pre[W];
         const int c = live[u] ? cell_of(i, pre, W) : 0;
         idx[u] = live[u] ? (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]) : 0;
         dcell[u] = c - pl.R;
@@ -107,7 +122,7 @@ __global__ __launch_bounds__(256) void k_gather_sinks(DevPlan pl, const float *_
         q[u] = live[u] ? slab_xyz(slab, n, idx[u], aos) : make_float3(0.f, 0.f, 0.f);
 #pragma unroll
     for (int u = 0; u < kGatherBlocks; u++) {
-        if (b0 + u >= pl.nsink_blocks) break;
+        if (b0 + u >= nsb) break;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (live[u]) {
             float z = q[u].z;
@@ -158,12 +173,13 @@ __device__ __forceinline__ float wave_max(float v) { return wave_reduce<true>(v)
 __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
                                                         float4 *__restrict__ box, float cellsize) {
     const int b0 = (blockIdx.x * 4 + (int)(threadIdx.x / kWave)) * kGatherBlocks;
-    if (b0 >= pl.nsrc_blocks) return;
+    const int nqb = src_blocks(pl);
+    if (b0 >= nqb || plan_bad(pl)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int W = pl.W;
     int par[kGatherBlocks];
 #pragma unroll
-    for (int u = 0; u < kGatherBlocks; u++) par[u] = pl.src_parent[min(b0 + u, pl.nsrc_blocks - 1)];
+    for (int u = 0; u < kGatherBlocks; u++) par[u] = pl.src_parent[min(b0 + u, nqb - 1)];
     uint64_t idx[kGatherBlocks];
     int cc[kGatherBlocks];
     bool live[kGatherBlocks];
@@ -175,7 +191,7 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
         pre[0] = 0;
         for (int c = 0; c < W; c++) pre[c + 1] = pre[c] + p.cell[c].N;
         const int i = (b0 + u) * kWave - pl.src_set_start[s] + lane;
-        live[u] = b0 + u < pl.nsrc_blocks && par[u] >= 0 && i < pre[W];
+        live[u] = b0 + u < nqb && par[u] >= 0 && i < pre[W];
         const int c = live[u] ? cell_of(i, pre, W) : 0;
         idx[u] = live[u] ? (uint64_t)p.cell[c].start + (uint64_t)(i - pre[c]) : 0;
         cc[u] = c;
@@ -189,7 +205,7 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
 #pragma unroll
     for (int u = 0; u < kGatherBlocks; u++) {
         const int b = b0 + u;
-        if (b >= pl.nsrc_blocks) break;
+        if (b >= nqb) break;
         if (par[u] < 0) continue;
         float4 v = make_float4(kFar, 0.f, 0.f, 0.f);
         const bool real = live[u];
@@ -245,7 +261,7 @@ __global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSl
                                                                float4 *__restrict__ box, float cpdf,
                                                                double invcpd) {
     const int b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
-    if (b >= pl.nsrc_blocks) return;
+    if (b >= src_blocks(pl) || plan_bad(pl)) return;
     const int s = cptr<int32_t>(pl.src_parent)[b];
     if (s < 0) return;  // a block no pencil covers is never read: left unwritten
     const int lane = threadIdx.x & (kWave - 1);
@@ -364,6 +380,7 @@ template <int SOFT>
 __global__ __launch_bounds__(64) void k_plain(DevPlan pl, PairConst pc, const float4 *__restrict__ sinks,
                                               const float4 *__restrict__ src, float4 *__restrict__ out) {
     const int blk = blockIdx.x;
+    if (blk >= sink_blocks(pl) || plan_bad(pl)) return;
     const int s = pl.parent[blk];
     const int id = blk * kWave + threadIdx.x;
     const float4 k = sinks[id];  // padded ids hold zeros (PencilKernel.cu:109-117)
@@ -439,7 +456,7 @@ __device__ __forceinline__ float4 apply_kick(float4 a, const KickArgs &k, int64_
 __global__ __launch_bounds__(256) void k_coadd_plain(DevPlan pl, const float4 *__restrict__ part,
                                                      float4 *__restrict__ acc, KickArgs kick) {
     const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
-    if (q >= pl.n_coadd_chunks) return;
+    if (q >= coadd_chunks(pl) || plan_bad(pl)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int cell = pl.coadd_chunk[2 * q], first = pl.coadd_chunk[2 * q + 1];
     const int j = cell / pl.Nk, z = cell % pl.Nk;
@@ -469,7 +486,7 @@ __global__ __launch_bounds__(256) void k_coadd_plain(DevPlan pl, const float4 *_
 __global__ __launch_bounds__(256) void k_coadd_f64(DevPlan pl, const double *__restrict__ part, double out_scale,
                                                    float4 *__restrict__ acc, KickArgs kick) {
     const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
-    if (q >= pl.n_coadd_chunks) return;
+    if (q >= coadd_chunks(pl) || plan_bad(pl)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int cell = pl.coadd_chunk[2 * q], first = pl.coadd_chunk[2 * q + 1];
     const int j = cell / pl.Nk, z = cell % pl.Nk;
@@ -618,6 +635,21 @@ __device__ __forceinline__ int tile_mode(const BatchCtx &cx, float gap2) {
     return gap2 <= cx.dens2 ? kDensity : kNoDensity;
 }
 
+// Pair interactions a wave has computed in each pair form (kNoDensity,
+// kDensity, kExact) and in short tails: wave-uniform, so the counts live in
+// scalar registers and cost no VALU work.  Counted only by the census
+// instantiations of k_batched: eight more 64-bit scalars push the single
+// spline's S = 2 kernel into SGPR and VGPR spills that cost it 7%.
+struct Census {
+    uint64_t form[3];
+    uint64_t tail;
+};
+__device__ __forceinline__ void census_add(Census &c, int mode, uint64_t n) {
+    c.form[0] += mode == kNoDensity ? n : 0;
+    c.form[1] += mode == kDensity ? n : 0;
+    c.form[2] += mode == kExact ? n : 0;
+}
+
 // One source tile (cnt <= 64 sources) against the S sinks
 // of every lane; sources are wave-uniform loads (scalar cache).  The force
 // terms of each batch of 8 sources are summed on their own before they join
@@ -699,8 +731,9 @@ __device__ __forceinline__ void mixed_tile(int mlo, int mhi, const NF_CONST floa
 
 // Register tile: 64*S sinks of one pencil on the lanes; sources are wave
 // uniform and come through the scalar cache.
-template <int SOFT, int S>
-__device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane) {
+template <int SOFT, int S, bool CENSUS>
+__device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane,
+                                          Census &cen) {
     const NF_CONST float4 *src = cptr<float4>(v.sources4);
     const NF_CONST float4 *box = cptr<float4>(v.box4);
     const NF_CONST int32_t *sink_set_start = cptr<int32_t>(v.sink_set_start);
@@ -711,6 +744,7 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
     const int s = w.sinkset;
     const int base = sink_set_start[s] + w.first;
     const int nsink = w.nsink;  // 64*S, or fewer for a masked tail tile
+    const uint64_t n_lo = (uint64_t)min(nsink, kWave * (S / 2)), n_hi = (uint64_t)nsink - n_lo;  // sinks per half
     const SinkCells sc = sink_cells(v, s);
     float kx[S], ky[S], kz[S];
     // The sinks' box: x and y over all slots, z per half of the slots (the
@@ -752,13 +786,11 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
     // Each block's FP32 tile sum joins an FP64 partial: a convert and an add
     // per component, which gfx950 issues at the FP32 rate (a TwoSum merge
     // into an FP32 (hi, lo) pair is 7 instructions and carries fewer bits).
-    double hx[S], hy[S], hz[S];
-    float hw[S];
+    // The FOF density w too: in a dense halo a sink meets thousands of pairs
+    // within b, and one FP32 running sum of them drifts by ~1e-5.
+    double hx[S], hy[S], hz[S], hw[S];
 #pragma unroll
-    for (int u = 0; u < S; u++) {
-        hx[u] = hy[u] = hz[u] = 0.0;
-        hw[u] = 0.f;
-    }
+    for (int u = 0; u < S; u++) hx[u] = hy[u] = hz[u] = hw[u] = 0.0;
 
     const int W = v.W;
     for (int y = 0; y < W; y++) {
@@ -801,6 +833,10 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
             const int mlo = pm & 3, mhi = pm >> 2;
             const NF_CONST float4 *sp = src + st + b * kWave;
             const int cnt = min(kWave, n - b * kWave);
+            if constexpr (CENSUS) {
+                census_add(cen, mlo, n_lo * (uint64_t)cnt);
+                census_add(cen, mhi, n_hi * (uint64_t)cnt);
+            }
             float tx[S], ty[S], tz[S], tw[S];
 #pragma unroll
             for (int u = 0; u < S; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
@@ -810,7 +846,7 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
                 hx[u] += (double)tx[u];
                 hy[u] += (double)ty[u];
                 hz[u] += (double)tz[u];
-                hw[u] += tw[u];
+                hw[u] += (double)tw[u];
             }
         }
     }
@@ -821,7 +857,7 @@ __device__ __forceinline__ void full_item(const BatchCtx &cx, const SicView &v, 
         o[0] = hx[u];
         o[1] = hy[u];
         o[2] = hz[u];
-        o[3] = (double)hw[u];
+        o[3] = hw[u];
     }
 }
 
@@ -844,13 +880,14 @@ __device__ __forceinline__ void group_two_sum(float &h, float &l) {
         l = l + l2 + e;
     }
 }
-__device__ __forceinline__ float group_sum(float v) {
+__device__ __forceinline__ double group_sum(double v) {
     for (int off = kGroupLanes / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
-template <int SOFT, int SU>
-__device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane) {
+template <int SOFT, int SU, bool CENSUS>
+__device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, const WorkItem &w, int lane,
+                                         Census &cen) {
     constexpr int TC = 16;  // 64-source blocks per TwoSum tile
     const NF_CONST float4 *box = cptr<float4>(v.box4);
     const NF_CONST int32_t *sink_set_start = cptr<int32_t>(v.sink_set_start);
@@ -913,6 +950,12 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
                     const int64_t gb = (int64_t)st / kWave + b;
                     const int mode =
                         tile_mode<SOFT>(cx, box_gap2(blx, sly, blz, bhx, shy, bhz, box[2 * gb], box[2 * gb + 1]));
+                    if constexpr (CENSUS) {
+                        const uint64_t pairs =
+                            (uint64_t)min(kGroups * SU, r - g0) * (uint64_t)min(kWave, n - b * kWave);
+                        census_add(cen, mode, pairs);
+                        cen.tail += pairs;
+                    }
                     // The block's 64 sources (padding included: exact zeros),
                     // 16 at a time on the group's lanes.
                     const float4 *bp = src + st + b * kWave + gl;
@@ -951,14 +994,14 @@ __device__ __forceinline__ void rem_item(const BatchCtx &cx, const SicView &v, c
             group_two_sum(hx[u], lx[u]);
             group_two_sum(hy[u], ly[u]);
             group_two_sum(hz[u], lz[u]);
-            const float ws = group_sum(hw[u]);
+            const double ws = group_sum((double)hw[u]);  // (an FP64 lane sum spills at S = 2)
             const int gi = g0 + grp * SU + u;
             if (gl == 0 && gi < r) {
                 NF_GLOBAL double *o = out + 4 * (int64_t)(base + gi);
                 o[0] = (double)hx[u] + (double)lx[u];
                 o[1] = (double)hy[u] + (double)ly[u];
                 o[2] = (double)hz[u] + (double)lz[u];
-                o[3] = (double)ws;
+                o[3] = ws;
             }
         }
     }
@@ -991,28 +1034,140 @@ int batched_kernel_waves(int softening, int sinks_per_lane) {
 namespace {
 
 // Persistent kernel: every wave pops items (of any SIC of the group) until
-// the queue is empty.
-template <int SOFT, int S, int WPE>
+// the queue is empty.  CENSUS: count the pair interactions of each pair form
+// into census (the diagnostic instantiations; the results are the same).
+// counter[0] is the queue; nitems < 0 means a list of at most -nitems items
+// whose length the prologue wrote into counter[1]; a census build adds its
+// counts into the four uint64 at counter + kCensusOffset.  One queue for
+// the whole grid: dealing the list to one queue per XCD, in interleaved
+// chunks of 8 to 128 consecutive items (which share source pencils), was
+// measured 11% slower.
+constexpr int kCensusOffset = 16;  // int32 units: 64 bytes into the counter's slot
+template <int SOFT, int S, int WPE, bool CENSUS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_batched(
-    BatchCtx cx, const SicView *__restrict__ views,
-                                                 const WorkItem *__restrict__ items, int32_t nitems,
-                                                 int32_t *__restrict__ counter) {
+    BatchCtx cx, const SicView *__restrict__ views, const WorkItem *__restrict__ items, int32_t nitems,
+    int32_t *__restrict__ counter) {
     const int lane = threadIdx.x & (kWave - 1);
+    if (nitems < 0) nitems = min(-nitems, *cptr<int32_t>(counter + 1));
+    Census cen = {{0, 0, 0}, 0};
     for (;;) {
         int it = 0;
         if (lane == 0) it = atomicAdd(counter, 1);
         it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= nitems) return;
+        if (it >= nitems) {
+            if (CENSUS && lane == 0) {
+                unsigned long long *census = reinterpret_cast<unsigned long long *>(counter + kCensusOffset);
+                for (int f = 0; f < 3; f++)
+                    if (cen.form[f]) atomicAdd(census + f, (unsigned long long)cen.form[f]);
+                if (cen.tail) atomicAdd(census + 3, (unsigned long long)cen.tail);
+            }
+            return;
+        }
         const WorkItem w = cptr<WorkItem>(items)[it];
         const int kind = w.kind_sic & 3;
         const SicView v = cptr<SicView>(views)[w.kind_sic >> 2];
-        if (kind == 0) full_item<SOFT, S>(cx, v, w, lane);
-        else if (kind == 2) full_item<SOFT, 1>(cx, v, w, lane);
-        else rem_item<SOFT, kRemSinks>(cx, v, w, lane);
+        if (kind == 0) full_item<SOFT, S, CENSUS>(cx, v, w, lane, cen);
+        else if (kind == 2) full_item<SOFT, 1, CENSUS>(cx, v, w, lane, cen);
+        else rem_item<SOFT, kRemSinks, CENSUS>(cx, v, w, lane, cen);
     }
 }
 
-__global__ void k_zero_counter(int32_t *c) { *c = 0; }
+// The batched launch's prologue (a fill kernel, so that it can sit in a
+// captured graph): clears the queue counter and, for a census build, the
+// census; for a device-planned list, copies its length into counter[1], or
+// 0 when the group holds a cell outside its slab.
+__global__ void k_zero_counter(int32_t *c, const int32_t *dnitems, const int32_t *dbad, int census) {
+    c[0] = 0;
+    if (dnitems) c[1] = (dbad && *dbad != 0) ? 0 : *dnitems;
+    if (census) {
+        unsigned long long *cs = reinterpret_cast<unsigned long long *>(c + kCensusOffset);
+        for (int f = 0; f < 4; f++) cs[f] = 0;
+    }
+}
+
+// Pair-math probe: the register-tile loop of k_batched alone (8-source batch
+// sums, 64-source FP32 tile sums accumulated in FP64), one pair form for
+// every block, on sinks and sources that stay in the caches.  Each wave holds
+// 64 S sinks and makes `passes` passes over `nsrc` sources (a multiple of
+// 64); lane 0 of each wave stores a checksum so that nothing is optimised
+// away.  Its rate is the ceiling of the tile loop at that tile and occupancy.
+template <int SOFT, int S, int MODE, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_probe(
+    PairConst pc, const float4 *__restrict__ src, int nsrc, int passes, double *__restrict__ out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave);
+    float kx[S], ky[S], kz[S];
+#pragma unroll
+    for (int u = 0; u < S; u++) {  // sinks spread over a cell-sized box around the sources
+        const int i = lane + kWave * u + 7 * wave;
+        kx[u] = 0.004f * (float)((i * 37) % 101) / 101.f - 0.002f;
+        ky[u] = 0.004f * (float)((i * 53) % 103) / 103.f - 0.002f;
+        kz[u] = 0.004f * (float)((i * 71) % 107) / 107.f - 0.002f;
+    }
+    double hx[S], hy[S], hz[S], hw[S];
+#pragma unroll
+    for (int u = 0; u < S; u++) hx[u] = hy[u] = hz[u] = hw[u] = 0.0;
+    const NF_CONST float4 *sp = cptr<float4>(src);
+    for (int p = 0; p < passes; p++)
+        for (int b = 0; b < nsrc; b += kWave) {
+            float tx[S], ty[S], tz[S], tw[S];
+#pragma unroll
+            for (int u = 0; u < S; u++) tx[u] = ty[u] = tz[u] = tw[u] = 0.f;
+            full_tile<SOFT, MODE, MODE, S>(sp + b, kWave, kx, ky, kz, pc, tx, ty, tz, tw);
+#pragma unroll
+            for (int u = 0; u < S; u++) {
+                hx[u] += (double)tx[u];
+                hy[u] += (double)ty[u];
+                hz[u] += (double)tz[u];
+                hw[u] += (double)tw[u];
+            }
+        }
+    double c = 0.0;
+#pragma unroll
+    for (int u = 0; u < S; u++) c += hx[u] + hy[u] + hz[u] + hw[u];
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) out[wave] = c;
+}
+
+template <int S, int MODE, int WPE>
+int run_probe_t(const PairConst &pc, double *rate) {
+    constexpr int kSrc = 1024, kPasses = 2;  // ~0.3 s per launch at S = 2 on a full GPU
+    int ncu = device_cu_count(), bpc = 0;
+    if (ncu <= 0) return (int)hipErrorNoDevice;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_probe<NF_SOFT_SINGLE_SPLINE, S, MODE, WPE>,
+                                                                256, 0);
+    if (e != hipSuccess) return (int)e;
+    const int grid = ncu * std::max(1, bpc);
+    float4 h[kSrc];
+    for (int i = 0; i < kSrc; i++)  // sources in the same box, none within the softening of a sink
+        h[i] = make_float4(0.004f * (float)((i * 29) % 97) / 97.f - 0.002f + 2e-5f,
+                           0.004f * (float)((i * 31) % 89) / 89.f - 0.002f + 2e-5f,
+                           0.004f * (float)((i * 43) % 83) / 83.f - 0.002f + 2e-5f, 0.f);
+    float4 *src = nullptr;
+    double *out = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    float ms = 0.f;
+    if ((e = hipMalloc(&src, sizeof h)) == hipSuccess && (e = hipMalloc(&out, sizeof(double) * grid * 4)) == hipSuccess &&
+        (e = hipMemcpy(src, h, sizeof h, hipMemcpyHostToDevice)) == hipSuccess &&
+        (e = hipEventCreate(&a)) == hipSuccess && (e = hipEventCreate(&b)) == hipSuccess) {
+        // a warm-up launch, then the timed one
+        hipLaunchKernelGGL((k_probe<NF_SOFT_SINGLE_SPLINE, S, MODE, WPE>), dim3(grid), dim3(256), 0, nullptr, pc, src,
+                           kSrc, kPasses, out);
+        (void)hipEventRecord(a, nullptr);
+        hipLaunchKernelGGL((k_probe<NF_SOFT_SINGLE_SPLINE, S, MODE, WPE>), dim3(grid), dim3(256), 0, nullptr, pc, src,
+                           kSrc, kPasses, out);
+        (void)hipEventRecord(b, nullptr);
+        e = hipEventSynchronize(b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (src) (void)hipFree(src);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return (int)e;
+    *rate = (double)grid * 4.0 * 64.0 * S * (double)kSrc * kPasses / (ms * 1e-3);
+    return 0;
+}
 
 template <int SOFT>
 int launch_plain_t(const DevPlan &pl, const PairConst &pc, const void *sinks4, const void *sources4, void *partial4,
@@ -1091,36 +1246,69 @@ int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale
     return (int)hipGetLastError();
 }
 
-template <int SOFT, int S>
-static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const SicView *views, const WorkItem *items,
-                            int32_t nitems, int32_t *counter, hipStream_t st) {
+template <int SOFT, int S, bool CENSUS>
+static int launch_batched_k(const BatchCtx &cx, const TileShape &ts, const SicView *views, const WorkItem *items,
+                            int32_t nitems, int32_t *counter, hipStream_t st, const int32_t *dnitems,
+                            const int32_t *dbad) {
+    constexpr auto kern = k_batched<SOFT, S, batched_waves_per_eu(SOFT, S), CENSUS>;
+    // Workgroups of 1, 2 or 4 waves; each wave pops its own items, so the
+    // size only changes how the dispatcher places the persistent grid.
+    const int wpw = ts.waves_per_wg == 1 || ts.waves_per_wg == 2 ? ts.waves_per_wg : 4;
+    const int threads = kWave * wpw;
     int blocks_per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, k_batched<SOFT, S, batched_waves_per_eu(SOFT, S)>, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kern, threads, 0) != hipSuccess ||
         blocks_per_cu <= 0)
         blocks_per_cu = 1;
-    if (ts.waves_per_cu > 0) blocks_per_cu = std::max(1, std::min(blocks_per_cu, ts.waves_per_cu / 4));
+    if (ts.waves_per_cu > 0) blocks_per_cu = std::max(1, std::min(blocks_per_cu, ts.waves_per_cu / wpw));
     const int ncu = device_cu_count();
     int grid = std::max(1, ncu) * blocks_per_cu;
-    const int need = (nitems + 3) / 4;
+    const int need = (nitems + wpw - 1) / wpw;
     if (grid > need) grid = std::max(1, need);
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL((k_batched<SOFT, S, batched_waves_per_eu(SOFT, S)>), dim3(grid), dim3(256), 0, st, cx, views, items, nitems, counter);
+    static_assert(kCensusOffset * 4 == kCensusBytes && kCensusBytes + 32 <= 256, "census inside the counter's slot");
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter, dnitems, dbad, CENSUS ? 1 : 0);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, st, cx, views, items, dnitems ? -nitems : nitems, counter);
     return (int)hipGetLastError();
+}
+
+template <int SOFT, int S>
+static int launch_batched_s(const BatchCtx &cx, const TileShape &ts, const SicView *views, const WorkItem *items,
+                            int32_t nitems, int32_t *counter, hipStream_t st, const int32_t *dnitems,
+                            const int32_t *dbad, bool census) {
+    if (census) return launch_batched_k<SOFT, S, true>(cx, ts, views, items, nitems, counter, st, dnitems, dbad);
+    return launch_batched_k<SOFT, S, false>(cx, ts, views, items, nitems, counter, st, dnitems, dbad);
 }
 
 template <int SOFT>
 static int launch_batched_soft(const BatchCtx &cx, const TileShape &ts, const SicView *views, const WorkItem *items,
-                               int32_t nitems, int32_t *counter, hipStream_t st) {
+                               int32_t nitems, int32_t *counter, hipStream_t st, const int32_t *dnitems,
+                               const int32_t *dbad, bool census) {
     switch (ts.sinks_per_lane) {
-        case 1: return launch_batched_s<SOFT, 1>(cx, ts, views, items, nitems, counter, st);
-        case 2: return launch_batched_s<SOFT, 2>(cx, ts, views, items, nitems, counter, st);
-        case 4: return launch_batched_s<SOFT, 4>(cx, ts, views, items, nitems, counter, st);
+        case 1: return launch_batched_s<SOFT, 1>(cx, ts, views, items, nitems, counter, st, dnitems, dbad, census);
+        case 2: return launch_batched_s<SOFT, 2>(cx, ts, views, items, nitems, counter, st, dnitems, dbad, census);
+        case 4: return launch_batched_s<SOFT, 4>(cx, ts, views, items, nitems, counter, st, dnitems, dbad, census);
         default: return (int)hipErrorInvalidValue;
     }
 }
 
+// The single-spline pair-math probe at (sinks per lane, pair form, waves per
+// SIMD): pair interactions per second over the whole GPU.
+int run_pair_probe(const PairConst &pc, int sinks_per_lane, int mode, int waves_per_simd, double *rate) {
+#define NF_PROBE(S_, M_, W_) \
+    if (sinks_per_lane == S_ && mode == M_ && waves_per_simd == W_) return run_probe_t<S_, M_, W_>(pc, rate);
+#define NF_PROBE_W(S_, M_) NF_PROBE(S_, M_, 4) NF_PROBE(S_, M_, 6) NF_PROBE(S_, M_, 8)
+#define NF_PROBE_M(S_) NF_PROBE_W(S_, kNoDensity) NF_PROBE_W(S_, kDensity) NF_PROBE_W(S_, kExact)
+    NF_PROBE_M(1)
+    NF_PROBE_M(2)
+    NF_PROBE_M(4)
+#undef NF_PROBE_M
+#undef NF_PROBE_W
+#undef NF_PROBE
+    return (int)hipErrorInvalidValue;
+}
+
 int launch_batched(const SicView *views, int cpd, int softening, const PairConst &pc, const TileShape &ts,
-                   const WorkItem *items, int32_t nitems, int32_t *counter, void *stream) {
+                   const WorkItem *items, int32_t nitems, int32_t *counter, void *stream, const int32_t *dnitems,
+                   const int32_t *dbad, bool census) {
     if (nitems == 0) return 0;
     hipStream_t st = static_cast<hipStream_t>(stream);
     BatchCtx cx;
@@ -1133,13 +1321,13 @@ int launch_batched(const SicView *views, int cpd, int softening, const PairConst
     // FP32 rounding of the box gap and of r^2.
     cx.dens2 = pc.b2x > 0.f ? (float)((double)pc.b2x * 1.01) : -1.f;
     switch (softening) {
-        case NF_SOFT_PLUMMER: return launch_batched_soft<NF_SOFT_PLUMMER>(cx, ts, views, items, nitems, counter, st);
+        case NF_SOFT_PLUMMER: return launch_batched_soft<NF_SOFT_PLUMMER>(cx, ts, views, items, nitems, counter, st, dnitems, dbad, census);
         case NF_SOFT_SINGLE_SPLINE:
-            return launch_batched_soft<NF_SOFT_SINGLE_SPLINE>(cx, ts, views, items, nitems, counter, st);
+            return launch_batched_soft<NF_SOFT_SINGLE_SPLINE>(cx, ts, views, items, nitems, counter, st, dnitems, dbad, census);
         case NF_SOFT_CUBIC_SPLINE:
-            return launch_batched_soft<NF_SOFT_CUBIC_SPLINE>(cx, ts, views, items, nitems, counter, st);
+            return launch_batched_soft<NF_SOFT_CUBIC_SPLINE>(cx, ts, views, items, nitems, counter, st, dnitems, dbad, census);
         case NF_SOFT_CUBIC_PLUMMER:
-            return launch_batched_soft<NF_SOFT_CUBIC_PLUMMER>(cx, ts, views, items, nitems, counter, st);
+            return launch_batched_soft<NF_SOFT_CUBIC_PLUMMER>(cx, ts, views, items, nitems, counter, st, dnitems, dbad, census);
         default: return (int)hipErrorInvalidValue;
     }
 }

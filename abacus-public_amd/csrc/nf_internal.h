@@ -44,7 +44,16 @@ struct SicView {
     int32_t pad;
 };
 
-// Device views of the plan arrays of one SIC.
+struct GPlanTotals;
+
+// Device views of the plan arrays of one SIC.  A SIC planned on the device
+// without a host round trip has its arrays laid out for upper bounds of its
+// sizes: then dtot (non-null) holds the actual block and chunk counts in
+// device memory, which the kernels read instead of nsink_blocks,
+// nsrc_blocks and n_coadd_chunks (those are the bounds: grid sizes and
+// buffer offsets), and dbad the launch group's count of planned cells that
+// lie outside their slabs (nonzero: every kernel of the group returns at
+// once, and the tasks fail).
 struct DevPlan {
     const int32_t *sink_set_start;
     const int32_t *sink_set_count;
@@ -61,6 +70,8 @@ struct DevPlan {
     int32_t nsink_sets, nsrc_sets, nsink_blocks, nsrc_blocks;
     int32_t W, R, cpd, Nk, j_width, j_low;
     int32_t n_coadd_chunks;
+    const GPlanTotals *dtot;
+    const int32_t *dbad;
 };
 
 // The slabs of one SIC on the device.
@@ -97,12 +108,14 @@ struct PairConst {
 
 int derive_pair_const(int softening, float eps, float b2, PairConst *pc);
 
-// Batched-path tuning knobs.
+// Batched-path tuning knobs.  Every shape gives bitwise the same partials:
+// a sink's sum does not depend on which lane, wave or workgroup computes it.
 struct TileShape {
     int sinks_per_lane;   // S: 1, 2 or 4
     int tile;             // TwoSum tile depth in sources (sinks on lanes)
     int rem_sinks;        // uniform sinks per pass in the remainder path
     int waves_per_cu;     // persistent grid size (0: from occupancy)
+    int waves_per_wg;     // workgroup size in waves: 1, 2 or 4 (0: 4)
 };
 
 // Kernel launchers (kernels.hip).  All enqueue on `stream` and return a HIP
@@ -117,15 +130,29 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool batched_stream, vo
 int launch_plain(const DevPlan &pl, int softening, const PairConst &pc, const void *sinks4,
                  const void *sources4, void *partial4, void *stream);
 int launch_coadd_plain(const DevPlan &pl, const void *partial4, float *acc4, const KickArgs &kick, void *stream);
-// One persistent launch over the work items of a group of SICs.
+// One persistent launch over the work items of a group of SICs.  With
+// dnitems (device memory) non-null the list holds *dnitems items and nitems
+// is only its bound; with dbad non-null and *dbad != 0 the kernel does
+// nothing.  counter names a 256-byte slot: the queue counter, the device
+// list's length and, with census, four uint64 at byte 64 (kCensusBytes)
+// receiving the pair interactions computed in each pair form (unsoftened,
+// with density, softened) and, in the fourth, those of short tails (also
+// counted by form), by a diagnostic build of the kernel.
+constexpr int kCensusBytes = 64;
 int launch_batched(const SicView *views, int cpd, int softening, const PairConst &pc, const TileShape &ts,
-                   const WorkItem *items, int32_t nitems, int32_t *counter, void *stream);
+                   const WorkItem *items, int32_t nitems, int32_t *counter, void *stream,
+                   const int32_t *dnitems = nullptr, const int32_t *dbad = nullptr, bool census = false);
 // Bounding boxes follow a padded source stream of nsrc_blocks blocks.
 inline void *box_after(void *sources4, int32_t nsrc_blocks) {
     return static_cast<char *>(sources4) + (size_t)nsrc_blocks * NF_BLOCK_SIZE * 16;
 }
 int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
                      const KickArgs &kick, void *stream);
+// The single-spline pair-math probe (the batched tile loop alone, one pair
+// form: 0 unsoftened without density, 1 with density, 2 the full spline) at
+// a tile of S sinks per lane and an occupancy of waves_per_simd (4, 6 or 8):
+// pair interactions per second over the whole GPU.  Synchronous.
+int run_pair_probe(const PairConst &pc, int sinks_per_lane, int mode, int waves_per_simd, double *rate);
 int device_cu_count();
 
 // ---- Device-side SIC planning (gpu_plan.hip) -----------------------------
@@ -158,17 +185,34 @@ struct GPlanOut {
     float *yoff;
     WorkItem *items[3];
     int32_t *hist, *cursor, *part;
+    const int32_t *group;  // the group's shared area (null: one SIC planned alone)
     int32_t sic;  // index of the SIC in its launch group (the work items' kind_sic)
 };
 constexpr int kPlanBuckets = 512;                   // 1/8-octave cost buckets of a device-built work list
-constexpr int kPlanSharedInts = 2 * 3 * kPlanBuckets;  // hist + cursor
+// The area a launch group's SICs share: hist, cursor, then the group's work
+// items per tile (S = 1, 2, 4) and its count of bad cells.
+constexpr int kPlanGroupItems = 2 * 3 * kPlanBuckets;
+constexpr int kPlanGroupBad = kPlanGroupItems + 3;
+constexpr int kPlanSharedInts = kPlanGroupBad + 1;
 size_t gpu_plan_part_ints(int nsink_sets, int nsrc_sets);
-// Phase A: the SIC's totals (sizes); `shared` (hist + cursor, or null) is
-// zeroed first.  Phase B, after the host laid the plan out: B1 for every SIC
-// of the group, then B2 for every SIC.
-int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *shared, void *stream);
+// Phase A: the SIC's totals (sizes), also added into the group's shared
+// area `group` (its items per tile and bad cells); `zero` (the shared area,
+// or null) is zeroed first, by the group's first SIC.  Phase B, with the
+// plan laid out: B1 for every SIC of the group, then B2 for every SIC; both
+// do nothing when the group has a bad cell.
+int launch_gpu_plan_a(const GPlanIn &in, GPlanTotals *tot, int32_t *zero, int32_t *group, void *stream);
 int launch_gpu_plan_b1(const GPlanIn &in, const GPlanOut &out, void *stream);
 int launch_gpu_plan_b2(const GPlanIn &in, const GPlanOut &out, void *stream);
+// The same phases for a whole launch group, one launch each: ins and outs
+// are the n SICs' arrays in device memory, tot their totals 256 bytes apart,
+// shared the group's shared area, first[s] SIC s's first workgroup in the
+// phase's grid of `grid` workgroups (gpu_plan_grid_a/b per SIC).  phase 0
+// zeroes the totals and the shared area, then counts (A); 1 and 2 are B1 and
+// B2.
+int gpu_plan_grid_a(const GPlanIn &in);
+int gpu_plan_grid_b(const GPlanIn &in);
+int launch_gpu_plan_group(int phase, const GPlanIn *ins, const GPlanOut *outs, GPlanTotals *tot, int32_t *shared,
+                          const int32_t *first, int n, int grid, void *stream);
 // The occupancy target (waves per SIMD) the batched kernel is built with for
 // a pair form and tile; the third template argument of k_batched.
 int batched_kernel_waves(int softening, int sinks_per_lane);

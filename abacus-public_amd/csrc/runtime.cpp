@@ -91,7 +91,9 @@ struct Task {
         bool on = false;
         const nf_cellinfo *cellinfo[NF_MAX_WIDTH] = {};
         uint64_t r0 = 0, r1 = 0;  // the SIC's sink rows in its slab
-        GPlanTotals tot{};        // phase A's sizes
+        GPlanTotals tot{};        // phase A's sizes, read back with the group's results
+        uint64_t chunks_bound = 0, items_bound = 0;  // bounds the plan is laid out for
+        uint64_t occ[4] = {};     // sinks in pencils of <64, 64-255, 256-1023, >=1024 sinks
     } dp;
     std::atomic<int> state{0};  // 0 queued/running, 1 done, <0 failed
     std::string error;
@@ -160,6 +162,8 @@ struct Worker {
     hipEvent_t gplan_ready = nullptr;  // the device-planned arrays and work lists are written
     DevBuf gscratch;                   // the device planner's totals and scratch
     PinBuf pin_tot;                    // phase A's totals, read back
+    PinBuf pin_gplan;                  // the group planning launches' inputs, staged
+    PinBuf pin_census;                 // the batched launch's interactions per pair form, read back
     hipGraphExec_t gexec = nullptr;  // the last group's executable graph (use_graphs)
     uint64_t graphs_instantiated = 0, graphs_updated = 0;
     DevBuf plan, work, slabs;
@@ -179,7 +183,8 @@ struct Runtime {
     bool stopping = false;
     int inflight = 0;   // groups taken by workers and not finished (under qm)
     bool warm = false;  // work has been flowing since the queue was last empty (under qm)
-    TileShape shape{2, 64, 4, 0};
+    TileShape shape{2, 64, 4, 0, 4};
+    std::vector<TileShape> candidates;  // the autotuner's search (empty: the shape is fixed)
     int max_group = 8;
     uint64_t group_interactions = 12000000000ull;
     bool autotune = false;
@@ -457,16 +462,43 @@ size_t task_bytes(const nf_sic &s, int path, int32_t n_chunks) {
 // row that wraps round the box appears twice), and an estimate of its pair
 // interactions for a uniform box, (2R+1)^2 n_sink n_window / cpd^2.  For
 // grouping decisions only.
-size_t dp_task_bytes(const nf_sic &s0, int path) {
-    nf_sic s = s0;
+// Sets s's block counts to those bounds; *chunks and *items get the bounds
+// of its co-add chunks and of its work items (at most one per padded sink
+// block plus one tail per pencil, for every tile).  The device planner's
+// plan and working set are laid out for these.
+void dp_bounds(nf_sic &s, uint64_t *chunks, uint64_t *items) {
     const uint64_t W = (uint64_t)s.nfwidth, cap = (uint64_t)INT32_MAX / NF_BLOCK_SIZE;
     uint64_t window = 0;
     for (int c = 0; c < s.nfwidth; c++) window += s.Nslab[c];
     const uint64_t wraps = ((uint64_t)(s.j_width + s.nfwidth - 1) + s.cpd - 1) / (uint64_t)s.cpd;
     s.NSinkBlocks = (int32_t)std::min<uint64_t>(cap, W * s.Nslab[s.nfradius] / NF_BLOCK_SIZE + s.NSinkSets);
     s.NSourceBlocks = (int32_t)std::min<uint64_t>(cap, wraps * window / NF_BLOCK_SIZE + s.NSourceSets);
-    const uint64_t chunks = std::min<uint64_t>(cap, s.Nslab[s.nfradius] / NF_BLOCK_SIZE + s.NSinkSets);
+    *chunks = std::min<uint64_t>(cap, s.Nslab[s.nfradius] / NF_BLOCK_SIZE + s.NSinkSets);
+    *items = (uint64_t)s.NSinkBlocks + (uint64_t)s.NSinkSets;
+}
+size_t dp_task_bytes(const nf_sic &s0, int path) {
+    nf_sic s = s0;
+    uint64_t chunks, items;
+    dp_bounds(s, &chunks, &items);
     return task_bytes(s, path, (int32_t)chunks);
+}
+
+// The sink-pencil occupancy histogram of a device-planned SIC, from the
+// central slab's cellinfo on the host (a running sum over each row): the
+// autotuner's key, before any plan exists.
+void dp_occupancy(const Task &t, uint64_t occ[4]) {
+    const nf_sic &s = t.sic;
+    const nf_cellinfo *ci = t.dp.cellinfo[s.nfradius];
+    for (int b = 0; b < 4; b++) occ[b] = 0;
+    for (int j = s.j_low; j < s.j_high; j++) {
+        const nf_cellinfo *row = ci + (size_t)j * s.cpd;
+        int64_t n = 0;
+        for (int c = 0; c < s.nfwidth; c++) n += std::max(row[c % s.cpd].count, 0);
+        for (int z = 0; z < s.cpd; z++) {
+            occ[n < 64 ? 0 : n < 256 ? 1 : n < 1024 ? 2 : 3] += (uint64_t)n;
+            n += std::max(row[(z + s.nfwidth) % s.cpd].count, 0) - std::max(row[z].count, 0);
+        }
+    }
 }
 uint64_t dp_interactions_estimate(const nf_sic &s) {
     double window = 0.0;
@@ -640,11 +672,17 @@ double event_ms(hipEvent_t a, hipEvent_t b) {
 
 // ---- Autotuner -------------------------------------------------------
 
-// Candidate tile shapes (sinks per lane; persistent waves from occupancy).
-// Fewer resident waves (8 per CU) never won on uniform or clustered slabs,
-// so they are not tried.
-const TileShape kCandidates[] = {{1, 64, 4, 0}, {2, 64, 4, 0}, {4, 64, 4, 0}};
-constexpr int kNumCandidates = sizeof(kCandidates) / sizeof(kCandidates[0]);
+// Candidate tile shapes: sinks per lane S (sinks per wave 64 S) times
+// waves per workgroup, each axis searched unless nf_setup fixes it (the
+// persistent grid always fills the occupancy the kernel was built for:
+// fewer resident waves, 8 per CU, never won on uniform or clustered slabs).
+std::vector<TileShape> tile_candidates(int fixed_s, int fixed_wpw) {
+    std::vector<TileShape> v;
+    for (int S : {1, 2, 4})
+        for (int wpw : {1, 2, 4})
+            if ((!fixed_s || S == fixed_s) && (!fixed_wpw || wpw == fixed_wpw)) v.push_back(TileShape{S, 64, 4, 0, wpw});
+    return v;
+}
 
 // The tuning key: a coarse histogram of sink-pencil occupancy (the fraction
 // of the group's sinks in pencils of <64, 64-255, 256-1023 and >=1024 sinks,
@@ -657,11 +695,11 @@ std::string occupancy_key(const std::vector<Task *> &g) {
     double tot = 0;
     uint64_t di = 0;
     for (Task *t : g) {
-        di += t->sic.DirectTotal;
+        di += queued_interactions(*t);  // device-planned: an estimate, before any plan exists
         if (t->dp.on) {
             for (int b = 0; b < 4; b++) {
-                h[b] += (double)t->dp.tot.occ[b];
-                tot += (double)t->dp.tot.occ[b];
+                h[b] += (double)t->dp.occ[b];
+                tot += (double)t->dp.occ[b];
             }
             continue;
         }
@@ -845,35 +883,48 @@ struct SlotRelease {
 // The device planner's scratch for a group of SICs: each SIC's totals
 // (256 bytes apiece, read back together), the work-list histogram and
 // cursors the group shares, each SIC's block sums.
+// Then the group-wide planning launches' inputs, uploaded from the worker's
+// pinned staging (laid out the same from o_in): the SICs' GPlanIn and
+// GPlanOut and the first workgroup of each SIC in phase A's and phase B's
+// grids.
 struct GPlanScratch {
     size_t o_shared = 0, total = 0;
     std::vector<size_t> o_part;
+    size_t o_in = 0, o_out = 0, o_fa = 0, o_fb = 0;
 };
 GPlanScratch gplan_scratch(const std::vector<Task *> &g) {
     static_assert(sizeof(GPlanTotals) <= 256, "totals slot");
     GPlanScratch L;
-    size_t o = 256 * g.size();
+    const size_t n = g.size();
+    size_t o = 256 * n;
     L.o_shared = o;
     o = align_up(o + sizeof(int32_t) * kPlanSharedInts, 256);
     for (Task *t : g) {
         L.o_part.push_back(o);
         o = align_up(o + sizeof(int32_t) * gpu_plan_part_ints(t->sic.NSinkSets, t->sic.NSourceSets), 256);
     }
-    L.total = o;
+    L.o_in = o;
+    L.o_out = align_up(L.o_in + sizeof(GPlanIn) * n, 256);
+    L.o_fa = align_up(L.o_out + sizeof(GPlanOut) * n, 256);
+    L.o_fb = align_up(L.o_fa + sizeof(int32_t) * (n + 1), 256);
+    L.total = align_up(L.o_fb + sizeof(int32_t) * (n + 1), 256);
     return L;
 }
 
 // Device planning, phase A, of a group of tasks from nf_submit_slab: their
 // cellinfo slabs go up through the slab cache (held in `cslot` until the
-// group is done), the counting kernels run on the worker's planning stream,
-// and the plans' sizes come back (one copy, one wait) into each t->sic.
-// Fills in[i] for phase B.
+// group is done) and the counting kernels are enqueued on the planning
+// stream.  Nothing waits for them: each SIC is laid out for the bounds of
+// its sizes (dp_bounds, set here), the kernels that follow read phase A's
+// counts from device memory, and the totals come back with the group's
+// results.  Fills in[i] for phase B.
 int plan_phase_a(Runtime &rt, Worker &w, std::vector<Task *> &g, std::vector<Runtime::Slot *> &cslot,
                  std::vector<GPlanIn> &in, const GPlanScratch &gs, uint64_t *to_dev) {
     const size_t n = g.size();
     int rc = w.gscratch.ensure(gs.total);
-    if (rc || (rc = w.pin_tot.ensure(256 * n))) return rc;
+    if (rc || (rc = w.pin_tot.ensure(256 * n)) || (rc = w.pin_gplan.ensure(gs.total - gs.o_in))) return rc;
     char *sc = static_cast<char *>(w.gscratch.p);
+    int32_t *shared = reinterpret_cast<int32_t *>(sc + gs.o_shared);
     in.assign(n, GPlanIn{});
     // Heavy register tiles are split against the group's estimated work (the
     // three planning phases all use this one threshold, so the item counts of
@@ -882,7 +933,8 @@ int plan_phase_a(Runtime &rt, Worker &w, std::vector<Task *> &g, std::vector<Run
     for (Task *t : g) group_est += dp_interactions_estimate(t->sic);
     for (size_t i = 0; i < n; i++) {
         Task &t = *g[i];
-        const nf_sic &s = t.sic;
+        nf_sic &s = t.sic;
+        dp_bounds(s, &t.dp.chunks_bound, &t.dp.items_bound);
         GPlanIn &gi = in[i];
         for (int ti = 0; ti < 3; ti++) gi.split_cost[ti] = tile_split_cost(s.softening, 1 << ti, group_est);
         gi.cpd = s.cpd;
@@ -901,17 +953,37 @@ int plan_phase_a(Runtime &rt, Worker &w, std::vector<Task *> &g, std::vector<Run
             gi.nslab[c] = s.Nslab[c];
             NF_HIP(hipStreamWaitEvent(w.pstream, sl->ready, 0));
         }
-        const int e = launch_gpu_plan_a(gi, reinterpret_cast<GPlanTotals *>(sc + 256 * i),
-                                        i == 0 ? reinterpret_cast<int32_t *>(sc + gs.o_shared) : nullptr, w.pstream);
-        if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
     }
-    NF_HIP(hipMemcpyAsync(w.pin_tot.p, sc, 256 * n, hipMemcpyDeviceToHost, w.pstream));
-    NF_HIP(hipStreamSynchronize(w.pstream));
+    // One launch per phase for the whole group: its SICs' inputs and grid
+    // offsets go up first.
+    char *pg = static_cast<char *>(w.pin_gplan.p);
+    std::memcpy(pg, in.data(), sizeof(GPlanIn) * n);
+    int32_t *fa = reinterpret_cast<int32_t *>(pg + (gs.o_fa - gs.o_in));
+    int grid = 0;
     for (size_t i = 0; i < n; i++) {
-        std::memcpy(&g[i]->dp.tot, static_cast<char *>(w.pin_tot.p) + 256 * i, sizeof(GPlanTotals));
-        if ((rc = apply_totals(g[i]->sic, g[i]->dp.tot))) return rc;
+        fa[i] = grid;
+        grid += gpu_plan_grid_a(in[i]);
     }
+    fa[n] = grid;
+    NF_HIP(hipMemcpyAsync(sc + gs.o_in, pg, sizeof(GPlanIn) * n, hipMemcpyHostToDevice, w.pstream));
+    NF_HIP(hipMemcpyAsync(sc + gs.o_fa, fa, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, w.pstream));
+    const int e = launch_gpu_plan_group(0, reinterpret_cast<const GPlanIn *>(sc + gs.o_in), nullptr,
+                                        reinterpret_cast<GPlanTotals *>(sc), shared,
+                                        reinterpret_cast<const int32_t *>(sc + gs.o_fa), (int)n, grid, w.pstream);
+    if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
     return NF_OK;
+}
+
+// The device planner's totals, read back with the group's results, into its
+// SICs' members; a cell outside its slab fails the group.
+int apply_group_totals(Worker &w, std::vector<Task *> &g) {
+    int rc = NF_OK;
+    for (size_t i = 0; i < g.size(); i++) {
+        std::memcpy(&g[i]->dp.tot, static_cast<char *>(w.pin_tot.p) + 256 * i, sizeof(GPlanTotals));
+        const int r = apply_totals(g[i]->sic, g[i]->dp.tot);
+        if (r && rc == NF_OK) rc = r;
+    }
+    return rc;
 }
 
 // Execute a group of compatible SICs on worker w: one plan upload, a gather
@@ -928,7 +1000,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
                   g[0]->sic.SlabId, g[0]->sic.j_low, g[0]->sic.j_high, g[0]->dp.on ? ", device-planned" : "");
     TraceRange trace_range(label);
     // Tasks from nf_submit_slab (grouped only with each other) are planned
-    // on the device: phase A first, since everything below needs the sizes.
+    // on the device: phase A is enqueued first; everything below is laid out
+    // for the bounds of the plans' sizes.
     const bool dplan = g[0]->dp.on;
     std::vector<GPlanIn> gin;
     GPlanScratch gs;
@@ -941,6 +1014,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         gs = gplan_scratch(g);
         const int rc = plan_phase_a(rt, w, g, cslot, gin, gs, &cell_bytes);
         if (rc) return rc;
+        if (path == NF_PATH_BATCHED && rt.autotune)
+            for (Task *t : g) dp_occupancy(*t, t->dp.occ);
     }
     const auto t_plan = clk::now();
     // Autotuning: a known occupancy histogram gets its tuned shape; a new one
@@ -985,7 +1060,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     std::vector<int> Svals;
     if (path == NF_PATH_BATCHED) {
         if (trial)
-            for (const TileShape &c : kCandidates)
+            for (const TileShape &c : rt.candidates)
                 if (std::find(Svals.begin(), Svals.end(), c.sinks_per_lane) == Svals.end()) Svals.push_back(c.sinks_per_lane);
         if (std::find(Svals.begin(), Svals.end(), ts.sinks_per_lane) == Svals.end()) Svals.push_back(ts.sinks_per_lane);
     }
@@ -997,7 +1072,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         itemsets.push_back(dplan ? std::vector<WorkItem>() : make_items(S));
         size_t m = itemsets.back().size();
         if (dplan)
-            for (Task *t : g) m += (size_t)t->dp.tot.items[S == 1 ? 0 : S == 2 ? 1 : 2];
+            for (Task *t : g) m += (size_t)t->dp.items_bound;
         nitems.push_back(m);
     }
     auto items_of = [&](int S) -> int {
@@ -1024,7 +1099,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     std::vector<PlanLayout> PL(n);
     for (int i = 0; i < n; i++) {
         PL[i] = plan_layout(g[i]->sic, plan_total,
-                            g[i]->dp.on ? (int32_t)g[i]->dp.tot.chunks : count_coadd_chunks(g[i]->sic));
+                            g[i]->dp.on ? (int32_t)g[i]->dp.chunks_bound : count_coadd_chunks(g[i]->sic));
         plan_total += PL[i].total;
     }
     // Device working set of each SIC.
@@ -1059,7 +1134,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     if (plan_total + work_total + slab_total > rt.max_buffer_bytes)
         return fail(NF_ERR_CAPACITY, "SIC group needs " + std::to_string(plan_total + work_total + slab_total) +
                                          " device bytes, buffer holds " + std::to_string(rt.max_buffer_bytes));
-    if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) ||
+    if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) || (rc = w.pin_census.ensure(32)) ||
         (rc = w.pin_plan.ensure(dplan ? sizeof(SicView) * (size_t)n : plan_total)))
         return rc;
     if (s0.mem == NF_MEM_HOST && (rc = w.slabs.ensure(slab_total))) return rc;
@@ -1094,6 +1169,11 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
 
         DevPlan &pl = plans[i];
         pl = dev_plan(s, L, pd);
+        if (dplan) {  // the actual counts, in device memory; the group's bad cells
+            char *sc = static_cast<char *>(w.gscratch.p);
+            pl.dtot = reinterpret_cast<const GPlanTotals *>(sc + 256 * (size_t)i);
+            pl.dbad = reinterpret_cast<const int32_t *>(sc + gs.o_shared) + kPlanGroupBad;
+        }
 
         DevSlabs &sl = slabs[i];
         for (int c = 0; c < s.nfwidth; c++) {
@@ -1154,14 +1234,29 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         std::vector<GPlanOut> go(n);
         for (int i = 0; i < n; i++) {
             go[i] = plan_out(pd, PL[i], i, shared, reinterpret_cast<int32_t *>(sc + gs.o_part[i]));
+            go[i].group = shared;
             for (size_t k = 0; k < Svals.size(); k++) {
                 const int ti = Svals[k] == 1 ? 0 : Svals[k] == 2 ? 1 : 2;
                 go[i].items[ti] = reinterpret_cast<WorkItem *>(pd + o_itemset[k]);
             }
         }
-        int e = 0;
-        for (int i = 0; i < n && !e; i++) e = launch_gpu_plan_b1(gin[i], go[i], w.pstream);
-        for (int i = 0; i < n && !e; i++) e = launch_gpu_plan_b2(gin[i], go[i], w.pstream);
+        char *pg = static_cast<char *>(w.pin_gplan.p);
+        std::memcpy(pg + (gs.o_out - gs.o_in), go.data(), sizeof(GPlanOut) * (size_t)n);
+        int32_t *fb = reinterpret_cast<int32_t *>(pg + (gs.o_fb - gs.o_in));
+        int grid = 0;
+        for (int i = 0; i < n; i++) {
+            fb[i] = grid;
+            grid += gpu_plan_grid_b(gin[i]);
+        }
+        fb[n] = grid;
+        NF_HIP(hipMemcpyAsync(sc + gs.o_out, pg + (gs.o_out - gs.o_in), sizeof(GPlanOut) * (size_t)n,
+                              hipMemcpyHostToDevice, w.pstream));
+        NF_HIP(hipMemcpyAsync(sc + gs.o_fb, fb, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyHostToDevice, w.pstream));
+        const GPlanIn *ins_d = reinterpret_cast<const GPlanIn *>(sc + gs.o_in);
+        const GPlanOut *outs_d = reinterpret_cast<const GPlanOut *>(sc + gs.o_out);
+        const int32_t *fb_d = reinterpret_cast<const int32_t *>(sc + gs.o_fb);
+        int e = launch_gpu_plan_group(1, ins_d, outs_d, nullptr, nullptr, fb_d, n, grid, w.pstream);
+        if (!e) e = launch_gpu_plan_group(2, ins_d, outs_d, nullptr, nullptr, fb_d, n, grid, w.pstream);
         if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
         NF_HIP(hipEventRecord(w.gplan_ready, w.pstream));
     } else {
@@ -1237,23 +1332,34 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     if (e) return fail(NF_ERR_HIP, std::string("gather launch: ") + hipGetErrorString((hipError_t)e));
     if (!use_graph) NF_HIP(hipEventRecord(w.ev[2], stream));
     std::vector<hipEvent_t> tev;
+    unsigned long long *census = nullptr;  // batched path: interactions per pair form (device)
     if (path == NF_PATH_BATCHED) {
         int32_t *counter = reinterpret_cast<int32_t *>(pd + o_counter);
+        if (rt.params.census)  // in the counter's 256-byte slot
+            census = reinterpret_cast<unsigned long long *>(pd + o_counter + kCensusBytes);
+        // A device-planned group's list lengths and bad cells are in its
+        // shared planning area.
+        const int32_t *gshared =
+            dplan ? reinterpret_cast<const int32_t *>(static_cast<char *>(w.gscratch.p) + gs.o_shared) : nullptr;
         auto launch = [&](const TileShape &shape) {
             const int k = items_of(shape.sinks_per_lane);
+            const int ti = shape.sinks_per_lane == 1 ? 0 : shape.sinks_per_lane == 2 ? 1 : 2;
             return launch_batched(reinterpret_cast<const SicView *>(pd + o_views), s0.cpd, s0.softening, pc, shape,
                                   reinterpret_cast<const WorkItem *>(pd + o_itemset[k]),
-                                  (int32_t)nitems[k], counter, stream);
+                                  (int32_t)nitems[k], counter, stream,
+                                  gshared ? gshared + kPlanGroupItems + ti : nullptr,
+                                  gshared ? gshared + kPlanGroupBad : nullptr, census != nullptr);
         };
         if (trial) {
             // Time each candidate; the chosen shape runs last (every candidate
             // writes bitwise the same partials: a sink's sum does not depend on
             // the sinks per lane).
-            tev.resize(kNumCandidates + 1);
+            const int ncand = (int)rt.candidates.size();
+            tev.resize(ncand + 1);
             for (auto &ev : tev) NF_HIP(hipEventCreate(&ev));
             NF_HIP(hipEventRecord(tev[0], stream));
-            for (int c = 0; c < kNumCandidates && !e; c++) {
-                e = launch(kCandidates[c]);
+            for (int c = 0; c < ncand && !e; c++) {
+                e = launch(rt.candidates[c]);
                 NF_HIP(hipEventRecord(tev[c + 1], stream));
             }
         }
@@ -1300,10 +1406,15 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         else w.graphs_instantiated++;
         NF_HIP(hipGraphLaunch(w.gexec, stream));
     }
+    // A device-planned group's totals and the batched launch's census come
+    // back behind its kernels.
+    if (dplan) NF_HIP(hipMemcpyAsync(w.pin_tot.p, w.gscratch.p, 256 * (size_t)n, hipMemcpyDeviceToHost, stream));
+    if (census) NF_HIP(hipMemcpyAsync(w.pin_census.p, census, 32, hipMemcpyDeviceToHost, stream));
     NF_HIP(hipEventRecord(w.ev[4], stream));
     if (enqueue_lock.owns_lock()) enqueue_lock.unlock();
     NF_HIP(hipEventSynchronize(w.ev[4]));
     on_gpu.stop();
+    if (dplan && (rc = apply_group_totals(w, g))) return rc;
     const double kern_ms = event_ms(w.ev[1], w.ev[4]);
     // A graph is timed as a whole (gathers and co-adds included).
     const double direct_ms = use_graph ? kern_ms : event_ms(w.ev[2], w.ev[3]);
@@ -1313,12 +1424,12 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     if (trial) {
         Runtime::TuneEntry te;
         int best = 0;
-        for (int c = 0; c < kNumCandidates; c++) {
+        for (int c = 0; c < (int)rt.candidates.size(); c++) {
             te.trial_ms.push_back(event_ms(tev[c], tev[c + 1]));
             if (te.trial_ms[c] < te.trial_ms[best]) best = c;
         }
         for (auto &ev : tev) (void)hipEventDestroy(ev);
-        te.shape = kCandidates[best];
+        te.shape = rt.candidates[best];
         for (Task *t : g) te.interactions += t->sic.DirectTotal;
         te.uses = 1;
         std::lock_guard<std::mutex> lk(rt.tune_m);
@@ -1389,6 +1500,12 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         st.sinks_per_lane = path == NF_PATH_BATCHED ? ts.sinks_per_lane : 0;
         st.autotune_trial = trial ? 1 : 0;
         st.waves_per_eu = path == NF_PATH_BATCHED ? batched_kernel_waves(s0.softening, ts.sinks_per_lane) : 0;
+        st.waves_per_wg = path == NF_PATH_BATCHED ? ts.waves_per_wg : 0;
+        if (census) {
+            const uint64_t *c = static_cast<const uint64_t *>(w.pin_census.p);
+            for (int f = 0; f < 3; f++) st.group_form_interactions[f] = c[f];
+            st.group_tail_interactions = c[3];
+        }
         st.device_planned = g[i]->dp.on ? 1 : 0;
         st.plan_s = g[i]->dp.on ? std::chrono::duration<double>(t_plan - t0).count() : 0.0;
     }
@@ -1495,6 +1612,8 @@ void destroy_worker(Worker &w) {
     w.plan.release();
     w.gscratch.release();
     w.pin_tot.release();
+    w.pin_gplan.release();
+    w.pin_census.release();
     w.work.release();
     w.slabs.release();
     w.pin_plan.release();
@@ -1566,13 +1685,15 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
     if (p->max_group < 0 || p->max_group > 4096) return fail(NF_ERR_INVALID, "max_group must be 0..4096");
     rt->max_group = p->max_group ? p->max_group : 8;
     if (p->group_interactions) rt->group_interactions = p->group_interactions;
-    if (p->sinks_per_lane != 0) {
-        if (p->sinks_per_lane != 1 && p->sinks_per_lane != 2 && p->sinks_per_lane != 4)
-            return fail(NF_ERR_INVALID, "sinks_per_lane must be 0, 1, 2 or 4");
-        rt->shape.sinks_per_lane = p->sinks_per_lane;
-    } else {
-        rt->autotune = true;  // the shape comes from the autotuner per occupancy histogram
-    }
+    if (p->sinks_per_lane != 0 && p->sinks_per_lane != 1 && p->sinks_per_lane != 2 && p->sinks_per_lane != 4)
+        return fail(NF_ERR_INVALID, "sinks_per_lane must be 0, 1, 2 or 4");
+    if (p->waves_per_wg != 0 && p->waves_per_wg != 1 && p->waves_per_wg != 2 && p->waves_per_wg != 4)
+        return fail(NF_ERR_INVALID, "waves_per_wg must be 0, 1, 2 or 4");
+    if (p->sinks_per_lane) rt->shape.sinks_per_lane = p->sinks_per_lane;
+    if (p->waves_per_wg) rt->shape.waves_per_wg = p->waves_per_wg;
+    // An axis left at 0 is searched by the autotuner per occupancy histogram.
+    rt->candidates = tile_candidates(p->sinks_per_lane, p->waves_per_wg);
+    rt->autotune = rt->candidates.size() > 1;
     if (p->source_tile != 0 && p->source_tile != 64) return fail(NF_ERR_INVALID, "source_tile must be 0 or 64");
     // Capacity per SIC, from the bytes a sink block brings with it (the
     // reference's estimate, GPUControl.cu:367-395, with this module's sizes):
@@ -1799,7 +1920,7 @@ extern "C" int nf_plan_sic_device(int32_t cpd, int32_t R, int32_t slab, int32_t 
     }
     GPlanTotals *tot_d = static_cast<GPlanTotals *>(b.scr);
     int32_t *part = reinterpret_cast<int32_t *>(static_cast<char *>(b.scr) + 256);
-    int e = launch_gpu_plan_a(in, tot_d, nullptr, b.st);
+    int e = launch_gpu_plan_a(in, tot_d, nullptr, nullptr, b.st);
     if (e) return fail(NF_ERR_HIP, std::string("device plan launch: ") + hipGetErrorString((hipError_t)e));
     GPlanTotals tot;
     NF_HIP(hipMemcpyAsync(&tot, tot_d, sizeof(tot), hipMemcpyDeviceToHost, b.st));
@@ -1962,12 +2083,14 @@ extern "C" int nf_autotune_table(char *buf, size_t len) {
             const size_t semi = kv.first.find(';');
             os << (first ? "" : ",") << "{\"occupancy_quarters\":[" << kv.first.substr(0, semi)
                << "],\"size_class\":" << kv.first.substr(semi + 1) << ",\"sinks_per_lane\":"
-               << e.shape.sinks_per_lane << ",\"waves_per_cu\":" << e.shape.waves_per_cu << ",\"uses\":" << e.uses
+               << e.shape.sinks_per_lane << ",\"waves_per_wg\":" << e.shape.waves_per_wg
+               << ",\"waves_per_cu\":" << e.shape.waves_per_cu << ",\"uses\":" << e.uses
                << ",\"interactions\":" << e.interactions << ",\"trial_ms\":[";
             for (size_t c = 0; c < e.trial_ms.size(); c++) os << (c ? "," : "") << e.trial_ms[c];
             os << "],\"candidates\":[";
-            for (int c = 0; c < kNumCandidates; c++)
-                os << (c ? "," : "") << "[" << kCandidates[c].sinks_per_lane << "," << kCandidates[c].waves_per_cu << "]";
+            for (size_t c = 0; c < g_rt->candidates.size(); c++)
+                os << (c ? "," : "") << "[" << g_rt->candidates[c].sinks_per_lane << ","
+                   << g_rt->candidates[c].waves_per_wg << "]";
             os << "]}";
             first = false;
         }
@@ -2157,5 +2280,19 @@ extern "C" int nf_execute_sic_noalloc(const nf_sic *sic, void *stream) {
     const hipError_t se = hipStreamSynchronize(st);
     if (e) return fail(NF_ERR_HIP, std::string("nf_execute_sic_noalloc launch: ") + hipGetErrorString((hipError_t)e));
     if (se != hipSuccess) return fail(NF_ERR_HIP, std::string("nf_execute_sic_noalloc: ") + hipGetErrorString(se));
+    return NF_OK;
+}
+
+extern "C" int nf_probe_pair_rate(float eps, float b2, int32_t sinks_per_lane, int32_t mode, int32_t waves_per_simd,
+                                  double *interactions_per_s) {
+    if (!interactions_per_s) return fail(NF_ERR_INVALID, "null argument");
+    if ((sinks_per_lane != 1 && sinks_per_lane != 2 && sinks_per_lane != 4) || mode < 0 || mode > 2 ||
+        (waves_per_simd != 4 && waves_per_simd != 6 && waves_per_simd != 8))
+        return fail(NF_ERR_INVALID, "probe shape: sinks_per_lane 1, 2 or 4; mode 0..2; waves_per_simd 4, 6 or 8");
+    PairConst pc;
+    int rc = derive_pair_const(NF_SOFT_SINGLE_SPLINE, eps, b2, &pc);
+    if (rc) return rc;
+    const int e = run_pair_probe(pc, sinks_per_lane, mode, waves_per_simd, interactions_per_s);
+    if (e) return fail(NF_ERR_HIP, std::string("pair probe: ") + hipGetErrorString((hipError_t)e));
     return NF_OK;
 }

@@ -84,6 +84,8 @@ class nf_task_stats(C.Structure):
         ("upload_s", C.c_double), ("download_s", C.c_double), ("slabs_uploaded", C.c_int32),
         ("slabs_reused", C.c_int32), ("sinks_per_lane", C.c_int32), ("autotune_trial", C.c_int32),
         ("waves_per_eu", C.c_int32), ("device_planned", C.c_int32), ("plan_s", C.c_double),
+        ("waves_per_wg", C.c_int32), ("group_form_interactions", C.c_uint64 * 3),
+        ("group_tail_interactions", C.c_uint64),
     ]
 
 
@@ -109,6 +111,7 @@ class nf_setup_params(C.Structure):
         ("buffers_per_device", C.c_int32), ("use_pinned", C.c_int32), ("path", C.c_int32),
         ("sinks_per_lane", C.c_int32), ("source_tile", C.c_int32), ("gpu_streams", C.c_int32),
         ("max_group", C.c_int32), ("use_graphs", C.c_int32), ("group_interactions", C.c_uint64),
+        ("waves_per_wg", C.c_int32), ("census", C.c_int32),
     ]
 
 
@@ -337,7 +340,9 @@ class Task:
     def stats(self):
         s = nf_task_stats()
         check(lib().nf_get_task_stats(self.h, C.byref(s)), "nf_get_task_stats")
-        return {k: getattr(s, k) for k, _ in s._fields_}
+        d = {k: getattr(s, k) for k, _ in s._fields_}
+        d["group_form_interactions"] = list(d["group_form_interactions"])
+        return d
 
     def release(self):
         if self.h:
@@ -357,7 +362,8 @@ class NearFieldDriver:
 
     def __init__(self, cpd, softening_length, b2, nfradius=2, softening="single_spline",
                  path="batched", buffers_per_device=3, max_buffer_bytes=None,
-                 sinks_per_lane=0, gpu_streams=0, max_group=0, use_graphs=0, group_interactions=0):
+                 sinks_per_lane=0, gpu_streams=0, max_group=0, use_graphs=0, group_interactions=0,
+                 waves_per_wg=0, census=0):
         from synth import eps_param
         if NearFieldDriver._active is not None:
             raise NFError("one NearFieldDriver per process (the GPU module is a singleton)")
@@ -375,7 +381,8 @@ class NearFieldDriver:
                             buffers_per_device=buffers_per_device, use_pinned=-1, path=PATHS[path],
                             sinks_per_lane=sinks_per_lane, source_tile=0, gpu_streams=gpu_streams,
                             max_group=max_group, use_graphs=use_graphs,
-                            group_interactions=int(group_interactions))
+                            group_interactions=int(group_interactions), waves_per_wg=waves_per_wg,
+                            census=census)
         msb, mrb = C.c_int32(0), C.c_int32(0)
         check(lib().nf_setup(C.byref(p), C.byref(msb), C.byref(mrb)), "nf_setup")
         NearFieldDriver._active = self

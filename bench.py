@@ -7,10 +7,11 @@ The whole box (PosXYZ slabs and float4 acceleration slabs) is resident in HBM;
 a step is one near-field evaluation of the whole box: every slab's
 SetInteractionCollection submitted through the C ABI (device-resident mode),
 gather + direct kernel + co-add on the GPU.  Every step plans each slab's SIC
-from the cellinfo (nf_plan_sic on the host) and submits it at once, so planning
-overlaps the GPU work already queued, as ExecuteSlabGPU does; with --planner
-device every step submits the slabs through nf_submit_slab, which plans them on
-the GPU.  5 timed steps by default; the line reports their total and their
+from the cellinfo inside the timed region, as ExecuteSlabGPU does: by default
+it submits the slabs through nf_submit_slab, which plans them on the GPU with
+no host round trip; with --planner host it plans each SIC with nf_plan_sic on
+the host and submits it at once, so planning overlaps the GPU work already
+queued.  5 timed steps by default; the line reports their total and their
 median.
 
 Prints one JSON line (rank 0).  Unit of work: unpadded pair interactions
@@ -75,14 +76,16 @@ def parse_args(argv=None):
     ap.add_argument("--graphs", type=int, default=None,
                     help="capture each group's kernels in a HIP graph (default: on for slab streams)")
     ap.add_argument("--paths", default="plain,batched")
-    ap.add_argument("--planner", default="host", choices=["host", "device", "host-once"],
-                    help="host: each step plans every slab's SIC with nf_plan_sic and submits it at once, so "
+    ap.add_argument("--planner", default="device", choices=["device", "host", "host-once"],
+                    help="device: each step submits the slabs with nf_submit_slab, which plans them on the GPU; "
+                         "host: each step plans every slab's SIC with nf_plan_sic and submits it at once, so "
                          "planning the next SIC overlaps the GPU work of the ones before (ExecuteSlabGPU's "
-                         "order); device: each step submits the slabs with nf_submit_slab, which plans them "
-                         "on the GPU; host-once: the SICs are planned once before timing and resubmitted "
+                         "order); host-once: the SICs are planned once before timing and resubmitted "
                          "every step (a comparison only: real steps re-plan, particles move)")
     ap.add_argument("--bpd", type=int, default=4, help="worker streams (DirectBPD)")
     ap.add_argument("--sinks-per-lane", type=int, default=0, help="0: the autotuner picks per occupancy histogram")
+    ap.add_argument("--waves-per-wg", type=int, default=0,
+                    help="batched workgroup size in waves, 1, 2 or 4 (0: the autotuner picks)")
     ap.add_argument("--max-group", type=int, default=8, help="SICs per persistent launch (batched path)")
     ap.add_argument("--group-interactions", type=float, default=0,
                     help="pair interactions after which a launch group takes no more SICs (0: the library's default)")
@@ -97,10 +100,16 @@ def parse_args(argv=None):
     ap.add_argument("--error-sample", type=float, default=0.01,
                     help="fraction of sinks checked against the FP64 oracle in the CPU leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    ap.add_argument("--no-census", action="store_true",
+                    help="whole-box workloads: skip the untimed census pass (share of pair interactions per "
+                         "pair form) after the batched path's timed steps")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU leg (the CPU baseline and the FP64-oracle error checks)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device-to-device twice (counter calibration) and exit")
+    ap.add_argument("--probe", action="store_true",
+                    help="only run the pair-math probe (nf_probe_pair_rate: the batched tile loop alone, per pair "
+                         "form, sinks per lane and occupancy) and print its table")
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="only print the HBM traffic table from rocprofv3 --pmc databases under DIR and exit")
     return ap.parse_args(argv)
@@ -113,6 +122,9 @@ def main():
         return
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
+        return
+    if args.probe:
+        pair_probe()
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -196,7 +208,7 @@ def run_resident(args, box, cpd, paths, steps, warmup, keep_acc=False):
     results = {}
     for path in paths:
         with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path=path, buffers_per_device=args.bpd,
-                                 max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
+                                 max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane, waves_per_wg=args.waves_per_wg,
                                  gpu_streams=args.gpu_streams, max_group=args.max_group,
                                  group_interactions=int(args.group_interactions),
                                  use_graphs=args.graphs or 0) as drv:
@@ -277,13 +289,47 @@ def run_resident(args, box, cpd, paths, steps, warmup, keep_acc=False):
                 results[path]["acc"] = {x: acc_d[x].to_host(np.float32, (box.slab(x).n, 4)) for x in range(cpd)}
                 results[path]["acc_sample"] = {x: results[path]["acc"][x][:2000] for x in (0, cpd // 2)}
             if path == "batched":
-                results[path]["autotune"] = [{k: e[k] for k in ("sinks_per_lane", "waves_per_cu", "uses")}
+                results[path]["autotune"] = [{k: e[k] for k in ("occupancy_quarters", "size_class", "sinks_per_lane",
+                                                                "waves_per_wg", "uses")}
                                              for e in drv.autotune_table()]
             log(f"[bench] {path}: {results[path]['interactions_per_s']:.4e} interactions/s wall, "
                 f"kernel {results[path]['kernel_interactions_per_s']:.4e}/s")
+        if path == "batched" and not args.no_census:
+            # One more, untimed pass with the census build of the kernel: the
+            # share of pair interactions in each pair form.
+            with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path=path, buffers_per_device=args.bpd,
+                                     max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
+                                     waves_per_wg=args.waves_per_wg, gpu_streams=args.gpu_streams,
+                                     max_group=args.max_group, group_interactions=int(args.group_interactions),
+                                     census=1) as cdrv:
+                tasks = [cdrv.submit(cdrv.plan(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr,
+                                               mem=nfm.NF_MEM_DEVICE)) for x in range(cpd)]
+                cst = []
+                for t in tasks:
+                    t.wait()
+                    cst.append(t.stats())
+                    t.release()
+            results[path]["pair_forms"] = pair_forms(cst)
+            log(f"[bench] {path}: pair forms {results[path]['pair_forms']}")
     for d in pos_d + acc_d:
         d.free()
     return results
+
+
+def pair_forms(stats):
+    """The batched launches' census over tasks (each task reports its launch
+    group's counts): the fraction of pair interactions computed in each pair
+    form and in short pencil tails, or None for the plain path."""
+    forms, tail = np.zeros(3), 0.0
+    for st in stats:
+        n = max(1, st["group_size"])
+        forms += np.asarray(st["group_form_interactions"], np.float64) / n
+        tail += st["group_tail_interactions"] / n
+    tot = forms.sum()
+    if tot <= 0:
+        return None
+    return {"unsoftened": forms[0] / tot, "density": forms[1] / tot, "softened": forms[2] / tot,
+            "short_tails": tail / tot}
 
 
 def summary_of(d):
@@ -524,7 +570,7 @@ def run_stream(args, box, workload, nsinks, steps, warmup, options=True):
 
     layout = nfm.NF_POS_AOS if aos is not None else nfm.NF_POS_XYZ
     with nfm.NearFieldDriver(cpd, soft_len, b2, nfradius=R, path="batched", buffers_per_device=args.bpd,
-                             max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane,
+                             max_buffer_bytes=6 << 30, sinks_per_lane=args.sinks_per_lane, waves_per_wg=args.waves_per_wg,
                              gpu_streams=args.gpu_streams, max_group=args.max_group, use_graphs=graphs,
                              group_interactions=int(args.group_interactions)) as drv:
         tp = time.time()
@@ -615,9 +661,38 @@ def stream_line(args, res):
             "plan_s_before_timing": res["plan_s"],
             "device_plan_s_per_step": sum(x["plan_s"] for x in stats) / steps,
         },
-        "autotune": [{k: e[k] for k in ("occupancy_quarters", "size_class", "sinks_per_lane", "uses")}
+        "autotune": [{k: e[k] for k in ("occupancy_quarters", "size_class", "sinks_per_lane", "waves_per_wg", "uses")}
                      for e in res["table"]],
     }
+
+
+def pair_probe():
+    """The pair-math probe of the single-spline forms at the uniform 512^3
+    workload's eps and b2: interactions/s of the batched tile loop alone for
+    each pair form (0 unsoftened, 1 with the FOF density, 2 the full spline),
+    sinks per lane and occupancy (waves per SIMD), and the fraction of the
+    FP32 peak at 20 FLOP per interaction."""
+    import ctypes as C
+    ell = 1.0 / 512
+    eps, _ = synth.eps_param("single_spline", ell / 40.0)
+    b2 = synth.fof_b2(ell)
+    lib = nfm.lib()
+    lib.nf_probe_pair_rate.argtypes = [C.c_float, C.c_float, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    rows = []
+    for S in (1, 2, 4):
+        for wpe in (4, 6, 8):
+            row = {"sinks_per_lane": S, "waves_per_simd": wpe}
+            for mode, name in ((0, "fast"), (1, "density"), (2, "exact")):
+                r = C.c_double(0.0)
+                nfm.check(lib.nf_probe_pair_rate(float(eps), float(b2), S, mode, wpe, C.byref(r)), "nf_probe_pair_rate")
+                row[name] = r.value
+            log(f"[probe] S={S} waves/SIMD={wpe}: fast {row['fast']:.3e} density {row['density']:.3e} "
+                f"exact {row['exact']:.3e} interactions/s")
+            rows.append(row)
+    best = max(rows, key=lambda r: r["fast"])
+    print(json.dumps({"pair_probe": rows, "peak_tflops": PEAK_FP32_TFLOPS, "flop_per_interaction": FLOP_PER_INTERACTION,
+                      "best_fast": best, "best_fast_frac_of_peak":
+                          FLOP_PER_INTERACTION * best["fast"] / (PEAK_FP32_TFLOPS * 1e12)}), flush=True)
 
 
 def calibrate_copy(mib):

@@ -182,8 +182,14 @@ typedef struct nf_task_stats {
     int32_t autotune_trial;        /* 1: its launch group timed every candidate tile */
     int32_t waves_per_eu;          /* batched path: occupancy target its kernel was built with */
     int32_t device_planned;        /* 1: submitted by nf_submit_slab (planned on the device) */
-    double plan_s;                 /* nf_submit_slab: worker wall time from the cellinfo
-                                      upload to the plan sizes being known on the host */
+    double plan_s;                 /* nf_submit_slab: worker wall time from the start of its
+                                      group to its planning kernels being enqueued */
+    int32_t waves_per_wg;          /* batched path: workgroup size of its launch, in waves */
+    uint64_t group_form_interactions[3]; /* batched path with nf_setup's census: pair
+                                            interactions of its launch group computed in the
+                                            unsoftened form without and with the FOF density,
+                                            and in the full softened form */
+    uint64_t group_tail_interactions;    /* of those, the ones in short pencil tails */
 } nf_task_stats;
 
 typedef struct nf_task nf_task;
@@ -213,6 +219,12 @@ typedef struct nf_setup_params {
     uint64_t group_interactions;  /* batched path: stop adding SICs to a launch group
                                      once it holds this many pair interactions
                                      (0: 1.2e10, about 4 ms of kernel time) */
+    int32_t waves_per_wg;         /* batched path: workgroup size of the persistent
+                                     launch, 1, 2 or 4 waves (0: auto) */
+    int32_t census;               /* batched path: 1 counts the pair interactions of each
+                                     pair form (nf_task_stats.group_form_interactions), with
+                                     a diagnostic build of the kernel some percent slower;
+                                     0: not counted (the fields read 0) */
 } nf_setup_params;
 
 /* GetNGPU (GPUControl.cu:267-271). */
@@ -291,14 +303,18 @@ typedef struct nf_throughput {
 } nf_throughput;
 int nf_get_throughput(nf_throughput *out);
 
-/* The batched-path autotuner's table (no counterpart in the reference).  With
- * sinks_per_lane = 0 the runtime keys every launch group on a coarse histogram
- * of its sink-pencil occupancies (the share of its sinks in pencils of <64,
- * 64-255, 256-1023 and >=1024 sinks, in quarters) and on its size class
- * (<2^31, <2^33 or more pair interactions); the first group with a new key
- * times each candidate tile shape on its own launch and the fastest is kept
- * for every later group with that key.  Writes the table as JSON into buf;
- * returns the length it needs (excluding the terminator), or < 0. */
+/* The batched-path autotuner's table (no counterpart in the reference).  The
+ * tile shape has two axes, sinks per lane (1, 2, 4: sinks per wave 64, 128,
+ * 256) and waves per workgroup (1, 2, 4); nf_setup's sinks_per_lane and
+ * waves_per_wg fix an axis when nonzero, and the autotuner searches the
+ * others.  It keys every launch group on a coarse histogram of its
+ * sink-pencil occupancies (the share of its sinks in pencils of <64, 64-255,
+ * 256-1023 and >=1024 sinks, in quarters) and on its size class (<2^31,
+ * <2^33 or more pair interactions); the first group with a new key times each
+ * candidate shape on its own launch and the fastest is kept for every later
+ * group with that key.  Every shape gives bitwise the same accelerations.
+ * Writes the table as JSON into buf; returns the length it needs (excluding
+ * the terminator), or < 0. */
 int nf_autotune_table(char *buf, size_t len);
 
 /* ---- host-slab streaming ---------------------------------------------
@@ -373,6 +389,17 @@ int nf_plan_sic_device(int32_t cpd, int32_t nfradius, int32_t slab, int32_t j_lo
                        const nf_cellinfo *const *cellinfo, const float *const *pos_slab,
                        const uint64_t *nslab, float eps, float b2, int32_t softening, void *arena,
                        uint64_t arena_bytes, nf_sic *out);
+
+/* ---- diagnostics --------------------------------------------------------
+ * No counterpart in the reference.  The pair-math probe: the batched
+ * kernel's register-tile loop alone (single spline; mode 0 the unsoftened
+ * form without the FOF density, 1 with it, 2 the full softened form) with
+ * sinks_per_lane 1, 2 or 4 and an occupancy of waves_per_simd 4, 6 or 8,
+ * over sources that stay in the caches: the ceiling of the pair math at that
+ * tile, in pair interactions per second over the whole GPU.  Synchronous, on
+ * the current device; allocates its own small buffers. */
+int nf_probe_pair_rate(float eps, float b2, int32_t sinks_per_lane, int32_t mode, int32_t waves_per_simd,
+                       double *interactions_per_s);
 
 /* ---- execution without allocation -------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

@@ -13,11 +13,16 @@ For each: rms and max relative error over the sample, outside
 softening-dominated sinks (softened pairs, r < e, carry more than half of
 |a|) and over sinks of condition number <= 100 (the sum of the pair terms'
 magnitudes over |a|), plus the batched path against the reference CPU directs
-(restated, oracle/cpu_directs.c) on the sample's sinks of two slabs.  The
-bars are those of the precision study on the 64^3 box: rms <= 1e-6 and, on
-sinks of condition number <= 100, max <= 1e-5 (FP32 pair terms alone err by
-~1e-7 x the condition number; tests/test_oracle.py pins that floor).
-NF_FULLSIZE_REPORT=<file> writes the numbers as JSON.
+(restated, oracle/cpu_directs.c) on the sample's sinks of two slabs, and the
+FP32 pair-term floor (oracle or_floor_f32: the batched kernel's pair terms,
+positions rounded into the pencil frame as it rounds them, summed exactly)
+on the same sinks.  The bars: rms <= 1e-6; max <= 1e-5 outside
+softening-dominated sinks, or within 1.5x of the floor's max where FP32 pair
+terms alone miss 1e-5 (they err by ~1e-7 x the condition number, more for a
+sink dominated by one close pair across a cell boundary, whose separation
+is formed from positions rounded into the pencil frame); on sinks of
+condition number <= 100, max <= 1e-5.  NF_FULLSIZE_REPORT=<file> writes the
+numbers as JSON.
 """
 import json
 import os
@@ -50,6 +55,11 @@ def _evaluate(name, box, accs, sample, cpu_slabs, eps, b2):
     well = keep & (cond <= 100.0)
     rep = {"sinks": int(sl.size), "softening_dominated": int((~keep).sum()),
            "condition_number_above_100": int((cond > 100).sum())}
+    floor = np.concatenate([po.floor_f32(box, R, eps, sl, ix, ce), ref[:, 3:]], axis=1)
+    rel_fl = rel_err(floor, ref)
+    rep["fp32_floor"] = {"outside_softening_dominated": summarize(rel_fl, keep, tail=True),
+                         "condition_number_le_100": summarize(rel_fl, well, tail=True),
+                         "max_error_over_condition_number": float((rel_fl[keep] / cond[keep]).max())}
     cpu = {x: po.execute_slab_cpu(box, R, "single_spline", eps, x)[0] for x in cpu_slabs}
     on_cpu = np.isin(sl, cpu_slabs)
     cpu_acc = np.zeros((int(on_cpu.sum()), 4), np.float32)
@@ -68,9 +78,15 @@ def _evaluate(name, box, accs, sample, cpu_slabs, eps, b2):
         rep[path] = {"all": summarize(rel, tail=True), "outside_softening_dominated": summarize(rel, keep, tail=True),
                      "condition_number_le_100": summarize(rel, well, tail=True),
                      "max_error_over_condition_number": float((rel[keep] / cond[keep]).max()),
-                     "vs_cpu_directs": summarize(rel_err(got[on_cpu], cpu_acc), keep[on_cpu], tail=True)}
-        # the FOF density against FP64 (self term included)
-        np.testing.assert_allclose(got[:, 3], ref[:, 3], rtol=3e-6, atol=1e-6 * float(b2))
+                     "vs_cpu_directs": summarize(rel_err(got[on_cpu], cpu_acc), keep[on_cpu], tail=True),
+                     "vs_fp32_floor": summarize(rel_err(got, floor), keep, tail=True)}
+        # the FOF density against FP64 (self term included): the batched path
+        # sums it in FP32 per 64-source tile and in FP64 across tiles; the
+        # plain path in FP32 throughout, as the reference does, which drifts
+        # by ~1e-5 for sinks with thousands of pairs within b in dense halos
+        rel_w = np.abs(got[:, 3].astype(np.float64) - ref[:, 3]) / np.maximum(np.abs(ref[:, 3]), float(b2))
+        rep[path]["density_w"] = {"rms": float(np.sqrt(np.mean(rel_w ** 2))), "max": float(rel_w.max())}
+        assert rel_w.max() <= (1e-5 if path == "batched" else 1e-4), (name, path, rep[path]["density_w"])
     REPORT[name] = rep
     _log(name, json.dumps(rep))
     out = os.environ.get("NF_FULLSIZE_REPORT")
@@ -81,10 +97,12 @@ def _evaluate(name, box, accs, sample, cpu_slabs, eps, b2):
 
 
 def _check(rep):
-    b, p = rep["batched"], rep.get("plain")
-    assert b["outside_softening_dominated"]["rms"] <= 1e-6, b
+    b, p, f = rep["batched"], rep.get("plain"), rep["fp32_floor"]
+    bo, fo = b["outside_softening_dominated"], f["outside_softening_dominated"]
+    assert bo["rms"] <= 1e-6, b
+    assert bo["max"] <= max(1e-5, 1.5 * fo["max"]), (b, f)
     assert b["condition_number_le_100"]["max"] <= 1e-5, b
-    assert b["max_error_over_condition_number"] <= 1e-6, b
+    assert b["max_error_over_condition_number"] <= max(1e-6, 1.5 * f["max_error_over_condition_number"]), (b, f)
     assert b["vs_cpu_directs"]["rms"] <= 1.5e-6, b
     if p is not None:
         assert p["outside_softening_dominated"]["rms"] <= 3e-6, p

@@ -18,14 +18,14 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def run_box(box, path, softening="single_spline", slabs=None, nsplit=1, sinks_per_lane=0, bpd=2):
+def run_box(box, path, softening="single_spline", slabs=None, nsplit=1, sinks_per_lane=0, bpd=2, waves_per_wg=0):
     """Run the listed slabs of a host-resident box; returns {slab: acc (n,4)}."""
     eps, b2, _ = params(box, softening)
     ell = box.mean_spacing()
     out = {}
     with nfm.NearFieldDriver(box.cpd, ell / 40.0, b2, nfradius=R, softening=softening, path=path,
                              buffers_per_device=bpd, max_buffer_bytes=2 << 30,
-                             sinks_per_lane=sinks_per_lane) as drv:
+                             sinks_per_lane=sinks_per_lane, waves_per_wg=waves_per_wg) as drv:
         assert drv.eps == eps
         slabs = range(box.cpd) if slabs is None else slabs
         for x in slabs:
@@ -112,21 +112,23 @@ def test_batched_register_tiles_agree(box9, ref9, S):
 
 
 def test_register_tiles_bitwise_equal(box9):
-    """A sink's sum does not depend on how many sinks a lane holds (S), on
-    which half of a tile it sits in (each half picks its own pair form), or
-    on a heavy register tile being split into one-wave tiles: S = 1, 2 and 4
-    give bitwise the same accelerations.  The heavy cell (1,500 particles)
-    makes register tiles costly enough to be split."""
+    """A sink's sum does not depend on the tile shape: how many sinks a lane
+    holds (S), how many waves share a workgroup, which half of a tile it
+    sits in (each half picks its own pair form), or a heavy register tile
+    being split into one-wave tiles.  All nine shapes (S = 1, 2, 4 times 1,
+    2, 4 waves per workgroup) give bitwise the same accelerations.  The heavy
+    cell (1,500 particles) makes register tiles costly enough to be split."""
     cpd = 7
     rng = np.random.default_rng(3)
     pts = rng.random((cpd ** 3 * 12, 3)) - 0.5
     heavy = synth.box_from_global(np.concatenate([pts, rng.normal(0.0, 0.01, (1500, 3))]), cpd)
+    shapes = [(S, w) for S in (1, 2, 4) for w in (1, 2, 4)]
     for box, slabs in ((box9, [0, 4]), (heavy, [2, 3, 4])):
-        res = {S: run_box(box, "batched", slabs=slabs, sinks_per_lane=S) for S in (1, 2, 4)}
+        res = {sh: run_box(box, "batched", slabs=slabs, sinks_per_lane=sh[0], waves_per_wg=sh[1]) for sh in shapes}
         for x in slabs:
-            assert np.isfinite(res[1][x]).all()
-            for S in (2, 4):
-                assert np.array_equal(res[1][x], res[S][x]), (box.cpd, x, S)
+            assert np.isfinite(res[(1, 4)][x]).all()
+            for sh in shapes:
+                assert np.array_equal(res[(1, 4)][x], res[sh][x]), (box.cpd, x, sh)
 
 
 def test_split_sics_equal_whole_slab(box9):
@@ -345,6 +347,37 @@ def test_launch_groups_match_single_launches(box9):
         assert np.array_equal(out[1][0][x], out[8][0][x]), x
 
 
+def test_census_counts_every_pair():
+    """The batched kernel's census of pair forms adds up to the launch
+    group's DirectTotal: every pair is computed exactly once, in one form.
+    With the FOF density off (b2 = 0) no pair takes the density form, and a
+    softening far larger than the cells puts every pair in the softened form.
+    The census build of the kernel gives bitwise the accelerations of the
+    plain build."""
+    box = small_uniform(cpd=7, ppc=20, seed=23)
+    ell = box.mean_spacing()
+    for soft_len, b2, check in ((ell / 40.0, synth.fof_b2(ell), None), (ell / 40.0, 0.0, "no_density"),
+                                (3.0 / box.cpd, synth.fof_b2(ell), "all_exact")):
+        accs = {}
+        with nfm.NearFieldDriver(box.cpd, soft_len, b2, path="batched", buffers_per_device=1,
+                                 max_buffer_bytes=1 << 30, max_group=1, census=1) as drv:
+            for x in (0, 3):
+                acc = accs[x] = np.full((box.slab(x).n, 4), np.nan, np.float32)
+                drv.execute_slab(box, x, acc, blocking=True)
+                for st in drv.finalize(x):
+                    f = st["group_form_interactions"]
+                    assert sum(f) == st["group_direct_total"] == st["direct_total"], (x, f, st["direct_total"])
+                    assert 0 < st["group_tail_interactions"] < st["direct_total"]
+                    if check == "no_density":
+                        assert f[1] == 0 and f[0] > 0, f
+                    if check == "all_exact":
+                        assert f[2] == st["direct_total"], f
+        if check is None:
+            plain_build = run_box(box, "batched", slabs=[0, 3], bpd=1)
+            for x in (0, 3):
+                assert np.array_equal(accs[x], plain_build[x]), x
+
+
 def test_autotuner_keys_on_occupancy(box9, ref9):
     eps, b2, _ = params(box9)
     with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
@@ -355,9 +388,11 @@ def test_autotuner_keys_on_occupancy(box9, ref9):
             drv.execute_slab(box9, x, accs[x], blocking=True)
             drv.finalize(x)
         table = drv.autotune_table()
-    assert table and all(e["sinks_per_lane"] in (1, 2, 4) for e in table)
+    assert table and all(e["sinks_per_lane"] in (1, 2, 4) and e["waves_per_wg"] in (1, 2, 4) for e in table)
     assert sum(e["uses"] for e in table) == 4
-    assert all(len(e["trial_ms"]) == len(e["candidates"]) for e in table)
+    # two axes searched: sinks per lane x waves per workgroup
+    assert all(len(e["candidates"]) == 9 and len(e["trial_ms"]) == 9 for e in table)
+    assert {tuple(c) for c in table[0]["candidates"]} == {(S, w) for S in (1, 2, 4) for w in (1, 2, 4)}
     # Every pencil holds 5 cells x ~20 sinks: one key, tuned once, reused.
     assert len(table) == 1 and table[0]["uses"] == 4, table
     assert table[0]["occupancy_quarters"] == [0, 4, 0, 0] and table[0]["size_class"] == 0
@@ -375,7 +410,8 @@ def test_graph_replays_match_eager(box9):
     slabs = [2, 3, 4]
     eager = run_box(box9, "batched", slabs=slabs, sinks_per_lane=2)
     with nfm.NearFieldDriver(box9.cpd, box9.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
-                             max_buffer_bytes=2 << 30, sinks_per_lane=2, max_group=1, use_graphs=1) as drv:
+                             max_buffer_bytes=2 << 30, sinks_per_lane=2, waves_per_wg=4, max_group=1,
+                             use_graphs=1) as drv:
         for rep in range(3):
             accs = {x: np.full((box9.slab(x).n, 4), np.nan, np.float32) for x in slabs}
             for x in slabs:
