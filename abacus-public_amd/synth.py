@@ -321,13 +321,13 @@ def _clustered_chunk(job):
     return [(x % cpd, box.slab(x)) for x in chunk]
 
 
-def clustered_slabs_parallel(n1d, cpd, seed, slabs, workers=None, chunk=4, **kw):
+def clustered_slabs_parallel(n1d, cpd, seed, slabs, workers=None, chunk=4, progress=None, **kw):
     """``clustered_slabs`` realised by a pool of forked processes, ``chunk``
     slabs per job: the same particles slab for slab (each slab's particles
     depend only on the seed), in a fraction of the time at production size
     (CPD 255, ~1e9 particles).  The GRF mesh is computed once, here, and
     inherited by the workers.  Forks: call it before the process touches a
-    GPU."""
+    GPU.  progress(done, total), if given, is called as jobs complete."""
     import multiprocessing as mp
     global _PAR_MESH
     want = sorted({x % cpd for x in slabs})
@@ -339,9 +339,11 @@ def clustered_slabs_parallel(n1d, cpd, seed, slabs, workers=None, chunk=4, **kw)
     out = [Slab(np.zeros(0, np.float32), np.zeros(cpd * cpd, np.int32)) for _ in range(cpd)]
     try:
         with mp.get_context("fork").Pool(workers) as pool:
-            for part in pool.imap_unordered(_clustered_chunk, jobs):
+            for done, part in enumerate(pool.imap_unordered(_clustered_chunk, jobs), 1):
                 for x, sl in part:
                     out[x] = sl
+                if progress:
+                    progress(done, len(jobs))
     finally:
         _PAR_MESH = None
     box = Box(cpd, out, f"clustered n1d={n1d} cpd={cpd} seed={seed} slabs {len(want)}/{cpd}")

@@ -517,7 +517,13 @@ def build_stream(workload, nsinks):
     _, cpd, n1d, _ = STREAM_WORKLOADS[workload]
     R = 2
     t0 = time.time()
-    box = synth.clustered_slabs_parallel(n1d, cpd, 20260255, range(-R, nsinks + R))
+    step = max(1, (nsinks + 2 * R) // 40)  # about ten progress lines
+
+    def progress(done, total):
+        if done % step == 0 or done == total:
+            log(f"[bench] stream slabs: {done}/{total} generator jobs done ({time.time() - t0:.0f}s)")
+
+    box = synth.clustered_slabs_parallel(n1d, cpd, 20260255, range(-R, nsinks + R), progress=progress)
     log(f"[bench] built {box.name}: {sum(s.n for s in box.slabs)} particles in {time.time() - t0:.1f}s")
     return box
 
