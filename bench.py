@@ -378,10 +378,11 @@ def resident_line(args, box, cpd, desc, results):
 
 def kernel_name(path, sinks_per_lane, waves_per_eu=0, softening=1):
     """The direct kernel's name as rocprofv3 prints it (template arguments:
-    pair form, sinks per lane, occupancy target in waves per SIMD)."""
+    pair form, sinks per lane, occupancy target in waves per SIMD, census
+    build: timed runs use the normal one)."""
     if path == "plain":
         return f"k_plain<{softening}>"
-    return f"k_batched<{softening}, {sinks_per_lane}, {waves_per_eu}>"
+    return f"k_batched<{softening}, {sinks_per_lane}, {waves_per_eu}, false>"
 
 
 def algorithmic_bytes(r, kd):

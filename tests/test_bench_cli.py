@@ -26,5 +26,20 @@ def test_help_lists_the_contract_options():
     r = run("--help")
     assert r.returncode == 0
     for opt in ("--gpus", "--steps", "--warmup", "--workload", "--paths", "--planner", "--no-cpu",
-                "--group-interactions", "--max-group", "--sinks-per-lane", "--gpu-streams", "--pmc-summary"):
+                "--group-interactions", "--max-group", "--sinks-per-lane", "--waves-per-wg", "--gpu-streams",
+                "--pmc-summary", "--probe", "--no-census", "--stream-slabs"):
         assert opt in r.stdout, opt
+
+
+def test_default_planner_is_the_device():
+    """The benchmark plans every SIC inside each timed step, on the device by
+    default (nf_submit_slab); host-once, which reuses plans across steps, is
+    only a comparison."""
+    sys.path.insert(0, ROOT)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_cli_probe", BENCH)
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    a = bench.parse_args([])
+    assert a.planner == "device" and a.gpus == 1 and a.workload == "uniform512"
+    assert bench.parse_args(["--planner", "host-once"]).planner == "host-once"
