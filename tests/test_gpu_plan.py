@@ -236,3 +236,31 @@ def test_shared_streams_match_per_worker_streams():
             assert np.isfinite(accs[x]).all(), key
             assert np.array_equal(accs[x], ref[x]), (key, x)
 
+
+
+def test_device_planned_graph_replays_match_eager():
+    """Device-planned launch groups captured in HIP graphs (the gathers,
+    persistent kernel and co-adds that read the planner's counts from device
+    memory), re-captured, updated in place and replayed over three passes with
+    eager work between them, give bitwise the eager host-planned results on
+    every pass, including a slab split into several SICs."""
+    box = small_uniform(cpd=9, ppc=20, seed=17)
+    eps, b2, _ = params(box)
+    slabs = [1, 4, 7]
+    with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=2,
+                             max_buffer_bytes=2 << 30, sinks_per_lane=2, waves_per_wg=4) as drv:
+        eager = run_sics(drv, box, slabs, nfm.NF_MEM_HOST)
+    with nfm.NearFieldDriver(box.cpd, box.mean_spacing() / 40.0, b2, path="batched", buffers_per_device=1,
+                             max_buffer_bytes=2 << 30, sinks_per_lane=2, waves_per_wg=4, max_group=4,
+                             use_graphs=1) as drv:
+        for rep in range(3):
+            got, st = run_sics(drv, box, slabs, nfm.NF_MEM_HOST, planner="device", with_stats=True)
+            assert all(s["graph"] == 1 and s["device_planned"] == 1 for s in st), rep
+            for x in slabs:
+                assert np.array_equal(got[x], eager[x]), (rep, x)
+            accs = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in slabs}
+            for x in slabs:  # eager work between the passes: a slab in three device-planned SICs
+                drv.execute_slab(box, x, accs[x], nsplit=3, planner="device")
+            for x in slabs:
+                drv.finalize(x)
+                assert np.array_equal(accs[x], eager[x]), (rep, x)
