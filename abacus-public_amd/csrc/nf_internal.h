@@ -3,6 +3,7 @@
 #ifndef NF_INTERNAL_H
 #define NF_INTERNAL_H
 
+#include <cstddef>
 #include <cstdint>
 #include <string>
 
@@ -12,6 +13,11 @@ namespace nf {
 
 // Record an error message for nf_last_error() on this thread; returns code.
 int fail(int code, const std::string &msg);
+
+// memcpy over nt threads, in 64-byte aligned chunks that cover every byte
+// (hostcopy.cpp); par_memcpy picks nt from the host's cores and the size.
+void par_memcpy_threads(void *dst, const void *src, size_t bytes, unsigned nt);
+void par_memcpy(void *dst, const void *src, size_t bytes);
 
 // Work item of the batched kernel: a run of `nsink` sinks starting at
 // `first` within sink pencil `sinkset`.  kind 0: register tile of 64*S sinks,

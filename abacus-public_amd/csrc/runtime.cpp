@@ -725,27 +725,6 @@ bool host_is_pinned(const void *p) {
     return e == hipSuccess && a.type == hipMemoryTypeHost;
 }
 
-// memcpy split over a few threads: one thread does not reach the host's
-// memory bandwidth, and a staged slab is tens of MB.
-void par_memcpy(void *dst, const void *src, size_t bytes) {
-    const size_t kMinChunk = 4u << 20;
-    unsigned nt = std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency() / 2));
-    nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, bytes / kMinChunk));
-    if (nt <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t chunk = (bytes / nt + 63) & ~size_t(63);
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < nt; t++) {
-        const size_t o = t * chunk;
-        if (o >= bytes) break;
-        th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, std::min(chunk, bytes - o)); });
-    }
-    std::memcpy(dst, src, std::min(chunk, bytes));
-    for (auto &x : th) x.join();
-}
-
 // Find the HBM copy of a host slab (inside a stream session) or upload it
 // into the least recently used free slot.  Returns the slot with a reference
 // held; *uploaded says whether it was copied now (its t0/t1 events then

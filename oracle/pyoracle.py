@@ -145,7 +145,8 @@ def error_report(box, R, kind, eps, b2, accs, frac, seed, slabs=None):
     outside softening-dominated sinks (pairs closer than the softening length
     carry more than half of |a_64|), and over the sinks outside those whose
     condition number (sum of the pair terms' magnitudes over |a_64|) is at
-    most 100."""
+    most 100; the sinks above 1e-5 also against the FP32 pair-term floor
+    (_above_floor)."""
     sl, ix, ce = sample_sinks(box, frac, seed, slabs)
     ref, soft, mag = direct_f64(box, R, kind, eps, b2, sl, ix, ce, with_soft=True, with_abs=True)
     an = np.linalg.norm(ref[:, :3], axis=1)
@@ -169,8 +170,29 @@ def error_report(box, R, kind, eps, b2, accs, frac, seed, slabs=None):
             got[m] = acc[int(x)][ix[m], :3]
         rel = np.linalg.norm(got - ref[:, :3], axis=1) / np.maximum(an, 1e-300)
         out[name] = {"all": stats(rel, np.ones(rel.size, bool)), "outside_softening_dominated": stats(rel, keep),
-                     "condition_number_le_100": stats(rel, keep & (cond <= 100))}
+                     "condition_number_le_100": stats(rel, keep & (cond <= 100)),
+                     "above_1e-5": _above_floor(box, R, eps, (sl, ix, ce), rel, ref, an, soft, cond, kind)}
     return out
+
+
+def _above_floor(box, R, eps, sample, rel, ref, an, soft, cond, kind, bar=1e-5, worst=6):
+    """The sinks of a result whose error exceeds ``bar``, against the FP32
+    pair-term floor on those sinks (floor_f32, single spline only): how many
+    err by no more than 1.5x the floor's own error, and the worst few with
+    their floor error, condition number and softened fraction of |a|."""
+    sl, ix, ce = sample
+    bad = np.flatnonzero(rel > bar)
+    rep = {"n": int(bad.size)}
+    if bad.size == 0 or kind_of(kind) != KIND["single_spline"]:
+        return rep
+    fl = floor_f32(box, R, eps, sl[bad], ix[bad], ce[bad])
+    fl_rel = np.linalg.norm(fl - ref[bad, :3], axis=1) / np.maximum(an[bad], 1e-300)
+    rep["within_1.5x_fp32_floor"] = int((rel[bad] <= 1.5 * fl_rel).sum())
+    order = np.argsort(-rel[bad])[:worst]
+    rep["worst"] = [{"slab": int(sl[bad[i]]), "index": int(ix[bad[i]]), "rel": float(rel[bad[i]]),
+                     "fp32_floor_rel": float(fl_rel[i]), "condition_number": float(cond[bad[i]]),
+                     "softened_fraction": float(soft[bad[i]] / max(an[bad[i]], 1e-300))} for i in order]
+    return rep
 
 
 def calc_di(box, R, x):

@@ -179,3 +179,45 @@ def test_slabstream255_fullsize_vs_fp64():
     rep = _evaluate("slabstream255", box, {"batched": acc}, po.sample_sinks(box, FRAC, seed=255, slabs=sinks),
                     [0, 5], eps, b2)
     _check(rep)
+
+
+def test_slabstream255_wrap_corners_vs_fp64():
+    """Every sink of the cells within two of both periodic wraps (rows and
+    columns 0-2 and 252-254) of sink slabs 206-210 of the CPD 255 stream,
+    slabs in pageable host memory (copied into pinned staging by several
+    threads).  Slab 208 holds 3,900,459 particles: 12 x that many bytes left
+    the last particle's z uncopied when the threaded copy's chunks were
+    rounded down, and the sinks around that particle, in these corner cells
+    of slabs 206-210, erred by up to 4e-2."""
+    sinks = list(range(206, 211))
+    _log("building the clustered CPD 255 stream (N=998^3 recipe, seed 20260255), sink slabs 206-210")
+    box = synth.clustered_slabs(998, 255, 20260255, range(sinks[0] - R, sinks[-1] + R + 1))
+    assert box.slab(208).n == 3900459
+    cpd = box.cpd
+    ell = box.mean_spacing()
+    eps, _ = synth.eps_param("single_spline", ell / 40.0)
+    b2 = synth.fof_b2(ell)
+    acc = {x: np.full((box.slab(x).n, 4), np.nan, np.float32) for x in sinks}
+    with nfm.NearFieldDriver(cpd, ell / 40.0, b2, nfradius=R, path="batched", buffers_per_device=4,
+                             max_buffer_bytes=6 << 30, gpu_streams=1, use_graphs=1) as drv:
+        drv.stream_begin()
+        tasks = [drv.submit(drv.plan(box, x, acc_ptr=acc[x].ctypes.data, mem=nfm.NF_MEM_HOST)) for x in sinks]
+        for t in tasks:
+            t.wait()
+            t.release()
+        drv.stream_end()
+    rows = [0, 1, 2, cpd - 3, cpd - 2, cpd - 1]
+    sl, ix, ce = [], [], []
+    for x in sinks:
+        s = box.slab(x)
+        for j in rows:
+            for k in rows:
+                c = j * cpd + k
+                n = int(s.count[c])
+                sl.append(np.full(n, x, np.int32))
+                ix.append(np.arange(s.start[c], s.start[c] + n, dtype=np.int64))
+                ce.append(np.full(n, c, np.int32))
+    sample = (np.concatenate(sl), np.concatenate(ix), np.concatenate(ce))
+    assert sample[0].size > 5000
+    rep = _evaluate("slabstream255_corners", box, {"batched": acc}, sample, [208], eps, b2)
+    _check(rep)

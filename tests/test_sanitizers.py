@@ -3,7 +3,9 @@ sanitizers: planner.cpp (nf_sic_arena_bytes, nf_plan_sic, nf_split_slab,
 built without HIP) and the oracle's liboracle.so and libcpudirects.so, each
 compiled with -fsanitize=address,undefined (undefined behaviour is fatal),
 then the planner and oracle CPU tests run against those builds in a child
-process with libasan preloaded.  A sanitizer report fails the test.
+process with libasan preloaded; and hostcopy.cpp's threaded copy (the copy of
+pageable slabs into pinned staging) checked byte for byte over sizes and
+thread counts.  A sanitizer report fails the test.
 CPU only: nothing here touches a GPU."""
 import os
 import shutil
@@ -67,3 +69,19 @@ def test_planner_and_oracle_under_sanitizers(sanitized_libs):
     assert "AddressSanitizer" not in r.stderr and "runtime error:" not in r.stderr, report
     assert r.returncode == 0, report
     assert " passed" in r.stdout, report
+
+
+def test_threaded_host_copy_covers_every_byte(tmp_path):
+    """par_memcpy_threads copies every byte and nothing past the end, for
+    sizes 0-4096 over 1-9 threads and slab-sized copies over 2-8 threads.
+    (Chunks of floor(bytes / threads) rounded up to 64 bytes once left the
+    last bytes of a 12 x 3,900,459-byte slab uncopied at 8 threads.)"""
+    if not shutil.which("g++"):
+        pytest.skip("g++ not found")
+    csrc = os.path.join(ROOT, "abacus-public_amd", "csrc")
+    exe = str(tmp_path / "copy_check")
+    _run(["g++", "-std=c++17", "-O1", "-pthread", *SAN, "-I", os.path.join(ROOT, "include"), "-I", csrc,
+          os.path.join(csrc, "hostcopy.cpp"), os.path.join(ROOT, "tests", "sanitize", "copy_check.cpp"), "-o", exe])
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="halt_on_error=1"))
+    assert r.returncode == 0 and "copied exactly" in r.stdout, (r.stdout[-2000:], r.stderr[-2000:])
