@@ -233,18 +233,17 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
 // takes an x-slice of each rather than all of one: tighter block boxes, so
 // more sink-block/source-block pairs clear the softening and density radii
 // and run the fast pair form.  Cells above kBinnedCellMax = 128 particles
-// keep the reference's order: ranking one costs a pass over the whole cell
-// per block it overlaps, and a replay of the mode split on a clustered box
+// keep the reference's order: a replay of the mode split on a clustered box
 // gained little from sorting them.
 // Cell offsets are added in FP64 (x + dc/cpd, rounded once).
 //
-// One wave per block: for each cell overlapping the block's 64 slots the
-// wave ranks the whole cell (ballots per bin) and the lanes whose rank falls
-// in the block write their source there; padding slots take kFar.  The box
-// is reduced over what the wave wrote.  All of the block's binned cells
-// (two 64-particle chunks each, in registers) load before any is ranked, so
-// a block costs one load latency.  (One block per wave measured faster than
-// 2, 4 or 8 walked in turn: more waves in flight.)
+// One wave per source pencil: the wave loads each of the pencil's binned
+// cells once (two 64-particle chunks each, all cells' loads in flight
+// together), ranks every particle within its cell (ballots per bin), writes
+// it to its slot, and reduces the box of each 64-source block of the pencil
+// over the particles that land in it (the pencil's slots run cell after
+// cell, so its blocks are met in order and each is written once).  The last
+// block's padding slots take kFar.
 constexpr int kXBins = 8;
 constexpr int kBinnedCellMax = 2 * kWave;
 
@@ -257,50 +256,90 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {  // set bits of m below
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// The box of the block being filled: each lane's running min/max over its
+// sources in the block, reduced across the wave once, when the next block
+// starts or the pencil ends.
+struct BlockBox {
+    int open = -1;
+    float mn[3], mx[3];
+    __device__ __forceinline__ void reset() {
+        mn[0] = mn[1] = mn[2] = __builtin_huge_valf();
+        mx[0] = mx[1] = mx[2] = -__builtin_huge_valf();
+    }
+    // Make b the open block (b >= open; the pencil's blocks are met in order).
+    __device__ __forceinline__ void to(int b, float4 *box, int64_t gb0, int lane) {
+        if (b == open) return;
+        flush(box, gb0, lane);
+        open = b;
+        reset();
+    }
+    __device__ __forceinline__ void add(bool in, float x, float y, float z) {
+        if (!in) return;
+        mn[0] = fminf(mn[0], x), mn[1] = fminf(mn[1], y), mn[2] = fminf(mn[2], z);
+        mx[0] = fmaxf(mx[0], x), mx[1] = fmaxf(mx[1], y), mx[2] = fmaxf(mx[2], z);
+    }
+    __device__ __forceinline__ void flush(float4 *box, int64_t gb0, int lane) {
+        if (open < 0) return;
+        const float4 lo = make_float4(wave_min(mn[0]), wave_min(mn[1]), wave_min(mn[2]), 0.f);
+        const float4 hi = make_float4(wave_max(mx[0]), wave_max(mx[1]), wave_max(mx[2]), 0.f);
+        if (lane == 0) {
+            box[2 * (gb0 + open)] = lo;
+            box[2 * (gb0 + open) + 1] = hi;
+        }
+    }
+};
+
 __global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
                                                                float4 *__restrict__ box, float cpdf,
                                                                double invcpd) {
-    const int b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
-    if (b >= src_blocks(pl) || plan_bad(pl)) return;
-    const int s = cptr<int32_t>(pl.src_parent)[b];
-    if (s < 0) return;  // a block no pencil covers is never read: left unwritten
+    const int s = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
+    if (s >= pl.nsrc_sets || plan_bad(pl)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int W = pl.W;
-    const float inf = __builtin_huge_valf();
     const nf_source_plan p = cptr<nf_source_plan>(pl.src_plan)[s];
-    const int l0 = b * kWave - cptr<int32_t>(pl.src_set_start)[s];  // the block's first slot in its pencil
     int pre[NF_MAX_WIDTH + 1];
     pre[0] = 0;
 #pragma unroll
     for (int c = 0; c < NF_MAX_WIDTH; c++) pre[c + 1] = pre[c] + (c < W ? p.cell[c].N : 0);
-    float4 *blk = out + (int64_t)b * kWave;
-    if (l0 + lane >= pre[NF_MAX_WIDTH]) blk[lane] = make_float4(kFar, 0.f, 0.f, 0.f);  // cells >= W hold 0
-    float mnx = inf, mny = inf, mnz = inf, mxx = -inf, mxy = -inf, mxz = -inf;
-    auto put = [&](int slot, float3 q, int c) {
-        const float x = (float)((double)q.x + (double)(c - pl.R) * invcpd);
-        blk[slot] = make_float4(x, q.y, q.z, 0.f);
-        mnx = fminf(mnx, x), mny = fminf(mny, q.y), mnz = fminf(mnz, q.z);
-        mxx = fmaxf(mxx, x), mxy = fmaxf(mxy, q.y), mxz = fmaxf(mxz, q.z);
-    };
-    auto overlaps = [&](int c) { return c < W && pre[c + 1] > pre[c] && pre[c + 1] > l0 && pre[c] < l0 + kWave; };
+    const int total = pre[NF_MAX_WIDTH];
+    if (total == 0) return;
+    const int start = cptr<int32_t>(pl.src_set_start)[s];  // the pencil's first slot, a multiple of 64
+    float4 *dst = out + start;
+    const int64_t gb0 = start / kWave;
+    const int last = (total - 1) / kWave;
+    if (last * kWave + lane >= total) dst[last * kWave + lane] = make_float4(kFar, 0.f, 0.f, 0.f);
     // the binned cells' particles (two chunks each) all load before any is ranked
     float3 q[NF_MAX_WIDTH][2];
 #pragma unroll
     for (int c = 0; c < NF_MAX_WIDTH; c++) {
         const int n = pre[c + 1] - pre[c];
         q[c][0] = q[c][1] = make_float3(0.f, 0.f, 0.f);
-        if (overlaps(c) && n <= kBinnedCellMax) {
+        if (n > 0 && n <= kBinnedCellMax) {
             const float *P = slab_ptr(sl, c);
             const uint64_t ns = slab_n(sl, c), first = p.cell[c].start;
             if (lane < n) q[c][0] = slab_xyz(P, ns, first + (uint64_t)lane, sl.aos);
             if (lane + kWave < n) q[c][1] = slab_xyz(P, ns, first + (uint64_t)(lane + kWave), sl.aos);
         }
     }
+    BlockBox bb;
+    // The sources at slots s0 (and s1, if in1) of this lane: into the stream
+    // and into the boxes of the blocks from slot lo to slot hi.
+    auto place = [&](int lo, int hi, bool in0, int s0, float3 v0, bool in1, int s1, float3 v1) {
+        if (in0) dst[s0] = make_float4(v0.x, v0.y, v0.z, 0.f);
+        if (in1) dst[s1] = make_float4(v1.x, v1.y, v1.z, 0.f);
+        for (int b = lo / kWave; b <= hi / kWave; b++) {
+            bb.to(b, box, gb0, lane);
+            bb.add(in0 && s0 / kWave == b, v0.x, v0.y, v0.z);
+            bb.add(in1 && s1 / kWave == b, v1.x, v1.y, v1.z);
+        }
+    };
+    auto shifted = [&](float3 v, int c) {
+        return make_float3((float)((double)v.x + (double)(c - pl.R) * invcpd), v.y, v.z);
+    };
 #pragma unroll
     for (int c = 0; c < NF_MAX_WIDTH; c++) {
-        if (!overlaps(c)) continue;
         const int n = pre[c + 1] - pre[c];
-        const int r0 = l0 - pre[c];  // the block's first slot as a rank in cell c (may be < 0)
+        if (n == 0) continue;
         if (n <= kBinnedCellMax) {
             const int bin0 = lane < n ? xbin(q[c][0].x, cpdf) : -1;
             const int bin1 = lane + kWave < n ? xbin(q[c][1].x, cpdf) : -1;
@@ -316,19 +355,21 @@ __global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSl
                     acc += __builtin_popcountll(m1);
                 }
             }
-            if (bin0 >= 0 && rank0 - r0 >= 0 && rank0 - r0 < kWave) put(rank0 - r0, q[c][0], c);
-            if (bin1 >= 0 && rank1 - r0 >= 0 && rank1 - r0 < kWave) put(rank1 - r0, q[c][1], c);
+            place(pre[c], pre[c + 1] - 1, bin0 >= 0, pre[c] + rank0, shifted(q[c][0], c), bin1 >= 0,
+                  pre[c] + rank1, shifted(q[c][1], c));
             continue;
         }
-        const int i = r0 + lane;  // a large cell: the reference's order
-        if (i >= 0 && i < n) put(lane, slab_xyz(slab_ptr(sl, c), slab_n(sl, c), p.cell[c].start + (uint64_t)i, sl.aos), c);
+        // a large cell: the reference's order, 64 at a time
+        const float *P = slab_ptr(sl, c);
+        const uint64_t ns = slab_n(sl, c), first = p.cell[c].start;
+        for (int i0 = 0; i0 < n; i0 += kWave) {
+            const bool in = i0 + lane < n;
+            const float3 v = in ? slab_xyz(P, ns, first + (uint64_t)(i0 + lane), sl.aos) : make_float3(0.f, 0.f, 0.f);
+            place(pre[c] + i0, pre[c] + min(n, i0 + kWave) - 1, in, pre[c] + i0 + lane, shifted(v, c), false, 0,
+                  v);
+        }
     }
-    mnx = wave_min(mnx), mny = wave_min(mny), mnz = wave_min(mnz);
-    mxx = wave_max(mxx), mxy = wave_max(mxy), mxz = wave_max(mxz);
-    if (lane == 0) {
-        box[2 * (int64_t)b] = make_float4(mnx, mny, mnz, 0.f);
-        box[2 * (int64_t)b + 1] = make_float4(mxx, mxy, mxz, 0.f);
-    }
+    bb.flush(box, gb0, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1205,8 +1246,8 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool batched_stream, vo
     if (pl.nsrc_blocks > 0) {
         const dim3 grid = grid_of(pl.nsrc_blocks);
         if (batched_stream)
-            hipLaunchKernelGGL(k_gather_sources_binned, grid_of(pl.nsrc_blocks, 1), dim3(256), 0, st, pl, sl,
-                               static_cast<float4 *>(sources4), box, (float)pl.cpd, invcpd);
+            hipLaunchKernelGGL(k_gather_sources_binned, dim3((unsigned)((pl.nsrc_sets + 3) / 4)), dim3(256), 0, st,
+                               pl, sl, static_cast<float4 *>(sources4), box, (float)pl.cpd, invcpd);
         else
             hipLaunchKernelGGL(k_gather_sources, grid, dim3(256), 0, st, pl, sl, static_cast<float4 *>(sources4),
                                box, cellsize);

@@ -95,8 +95,9 @@ def parse_args(argv=None):
                     help="whole-box workloads: skip the other workloads' lines (clustered 256^3 plain and "
                          "batched, the CPD 255 slab stream) that the default run adds under 'configs'")
     ap.add_argument("--config-steps", type=int, default=5, help="timed steps of each 'configs' workload")
-    ap.add_argument("--config-stream-slabs", type=int, default=24,
-                    help="sink slabs per pass of the slab stream under 'configs'")
+    ap.add_argument("--config-stream-slabs", type=int, default=255,
+                    help="sink slabs per pass of the slab stream under 'configs' (255: every slab of the "
+                         "CPD 255 box, ~1e9 particles, ~28 GB of host memory)")
     ap.add_argument("--error-sample", type=float, default=0.01,
                     help="fraction of sinks checked against the FP64 oracle in the CPU leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
