@@ -289,12 +289,9 @@ struct BlockBox {
     }
 };
 
-__global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
-                                                               float4 *__restrict__ box, float cpdf,
-                                                               double invcpd) {
-    const int s = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
-    if (s >= pl.nsrc_sets || plan_bad(pl)) return;
-    const int lane = threadIdx.x & (kWave - 1);
+__device__ __forceinline__ void gather_pencil_binned(const DevPlan &pl, const DevSlabs &sl, float4 *__restrict__ out,
+                                                     float4 *__restrict__ box, float cpdf, double invcpd, int s,
+                                                     int lane) {
     const int W = pl.W;
     const nf_source_plan p = cptr<nf_source_plan>(pl.src_plan)[s];
     int pre[NF_MAX_WIDTH + 1];
@@ -370,6 +367,41 @@ __global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSl
         }
     }
     bb.flush(box, gb0, lane);
+}
+
+__global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
+                                                               float4 *__restrict__ box, float cpdf,
+                                                               double invcpd) {
+    const int s = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
+    if (s >= pl.nsrc_sets || plan_bad(pl)) return;
+    gather_pencil_binned(pl, sl, out, box, cpdf, invcpd, s, threadIdx.x & (kWave - 1));
+}
+
+// The SIC of a group-wide launch that wave w belongs to (first[] ascending,
+// first[n] the total; a group holds a few SICs).
+__device__ __forceinline__ int group_sic_of(const int32_t *first, int n, int w) {
+    const NF_CONST int32_t *f = cptr<int32_t>(first);
+    int i = 0;
+    while (i < n - 1 && w >= f[i + 1]) i++;
+    return i;
+}
+
+// Every SIC's source gather of a launch group in one launch: a group of
+// small SICs (a clustered box's) fills the GPU where one SIC's pencils do not.
+__global__ __launch_bounds__(256) void k_gather_sources_binned_group(const GroupSic *__restrict__ g,
+                                                                     const int32_t *__restrict__ first, int n,
+                                                                     float cpdf, double invcpd) {
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
+    if (w >= cptr<int32_t>(first)[n]) return;
+    const int i = group_sic_of(first, n, w);
+    const DevPlan pl = cptr<GroupSic>(g)[i].pl;
+    const DevSlabs sl = cptr<GroupSic>(g)[i].sl;
+    void *const sources4 = cptr<GroupSic>(g)[i].sources4;
+    const int s = w - cptr<int32_t>(first)[i];
+    if (s >= pl.nsrc_sets || plan_bad(pl)) return;
+    float4 *src = static_cast<float4 *>(sources4);
+    gather_pencil_binned(pl, sl, src, src + (size_t)pl.nsrc_blocks * NF_BLOCK_SIZE, cpdf, invcpd, s,
+                         threadIdx.x & (kWave - 1));  // the block boxes follow the stream
 }
 
 // ---------------------------------------------------------------------------
@@ -524,11 +556,8 @@ __global__ __launch_bounds__(256) void k_coadd_plain(DevPlan pl, const float4 *_
 
 // FP64 co-add: sum of the W double partials, scaled back to the reference's
 // units (out_scale applies to a, not to w).  One wave per 64-particle chunk.
-__global__ __launch_bounds__(256) void k_coadd_f64(DevPlan pl, const double *__restrict__ part, double out_scale,
-                                                   float4 *__restrict__ acc, KickArgs kick) {
-    const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
-    if (q >= coadd_chunks(pl) || plan_bad(pl)) return;
-    const int lane = threadIdx.x & (kWave - 1);
+__device__ __forceinline__ void coadd_chunk_f64(const DevPlan &pl, const double *__restrict__ part, double out_scale,
+                                                float4 *__restrict__ acc, const KickArgs &kick, int q, int lane) {
     const int cell = pl.coadd_chunk[2 * q], first = pl.coadd_chunk[2 * q + 1];
     const int j = cell / pl.Nk, z = cell % pl.Nk;
     int64_t base[NF_MAX_WIDTH];
@@ -548,6 +577,27 @@ __global__ __launch_bounds__(256) void k_coadd_f64(DevPlan pl, const double *__r
     }
     const float4 a = make_float4((float)(a0 * out_scale), (float)(a1 * out_scale), (float)(a2 * out_scale), (float)a3);
     acc[cstart + i] = apply_kick(a, kick, (int64_t)cstart + i);
+}
+
+__global__ __launch_bounds__(256) void k_coadd_f64(DevPlan pl, const double *__restrict__ part, double out_scale,
+                                                   float4 *__restrict__ acc, KickArgs kick) {
+    const int q = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+    if (q >= coadd_chunks(pl) || plan_bad(pl)) return;
+    coadd_chunk_f64(pl, part, out_scale, acc, kick, q, threadIdx.x & (kWave - 1));
+}
+
+// Every SIC's FP64 co-add of a launch group in one launch.
+__global__ __launch_bounds__(256) void k_coadd_f64_group(const GroupSic *__restrict__ g,
+                                                         const int32_t *__restrict__ first, int n) {
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
+    if (w >= cptr<int32_t>(first)[n]) return;
+    const int i = group_sic_of(first, n, w);
+    const DevPlan pl = cptr<GroupSic>(g)[i].pl;
+    const int q = w - cptr<int32_t>(first)[i];
+    if (q >= coadd_chunks(pl) || plan_bad(pl)) return;
+    const KickArgs kick = cptr<GroupSic>(g)[i].kick;
+    coadd_chunk_f64(pl, static_cast<const double *>(cptr<GroupSic>(g)[i].partial), cptr<GroupSic>(g)[i].out_scale,
+                    reinterpret_cast<float4 *>(cptr<GroupSic>(g)[i].acc4), kick, q, threadIdx.x & (kWave - 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -1252,6 +1302,20 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool batched_stream, vo
             hipLaunchKernelGGL(k_gather_sources, grid, dim3(256), 0, st, pl, sl, static_cast<float4 *>(sources4),
                                box, cellsize);
     }
+    return (int)hipGetLastError();
+}
+
+int launch_gather_group(const GroupSic *g, const int32_t *first, int n, int total, int cpd, void *stream) {
+    if (total <= 0) return 0;
+    hipLaunchKernelGGL(k_gather_sources_binned_group, dim3((unsigned)((total + 3) / 4)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), g, first, n, (float)cpd, 1.0 / (double)cpd);
+    return (int)hipGetLastError();
+}
+
+int launch_coadd_group(const GroupSic *g, const int32_t *first, int n, int total, void *stream) {
+    if (total <= 0) return 0;
+    hipLaunchKernelGGL(k_coadd_f64_group, dim3((unsigned)((total + 3) / 4)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), g, first, n);
     return (int)hipGetLastError();
 }
 

@@ -154,6 +154,27 @@ inline void *box_after(void *sources4, int32_t nsrc_blocks) {
 }
 int launch_coadd_f64(const DevPlan &pl, const void *partial_d4, double out_scale, float *acc4,
                      const KickArgs &kick, void *stream);
+// One SIC of a launch group, for the group-wide batched-path gather and
+// co-add launches (device memory, beside the group's view table).
+struct GroupSic {
+    DevPlan pl;
+    DevSlabs sl;
+    KickArgs kick;
+    void *sources4;      // padded source stream; its block boxes follow it
+    const void *partial;  // FP64 partials, 4 per padded sink slot
+    float *acc4;
+    double out_scale;
+};
+// Waves of each SIC in the group-wide launches: the gather takes one per
+// source pencil, the co-add one per 64-particle chunk (its bound, for a
+// device-planned SIC).
+inline int32_t group_gather_waves(const DevPlan &pl) { return pl.nsrc_sets; }
+inline int32_t group_coadd_waves(const DevPlan &pl) { return pl.n_coadd_chunks; }
+// The batched-path source gathers of n SICs as one launch, then (a second
+// call) their FP64 co-adds as one launch.  first[i] is the first wave of SIC
+// i and first[n] the total (device memory, like g).
+int launch_gather_group(const GroupSic *g, const int32_t *first, int n, int total, int cpd, void *stream);
+int launch_coadd_group(const GroupSic *g, const int32_t *first, int n, int total, void *stream);
 // The single-spline pair-math probe (the batched tile loop alone, one pair
 // form: 0 unsoftened without density, 1 with density, 2 the full spline) at
 // a tile of S sinks per lane and an occupancy of waves_per_simd (4, 6 or 8):

@@ -1062,9 +1062,14 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     size_t items_total = 0;
     for (size_t v : nitems) items_total += align_up(sizeof(WorkItem) * std::max<size_t>(v, 1), 256);
 
-    // Plan upload: [views][items][counter][plan arrays of each SIC].
+    // Plan upload: [views][group SICs][first waves][items][counter][plan
+    // arrays of each SIC]; a device-planned group uploads the head (up to the
+    // items) only.
     const size_t o_views = 0;
-    const size_t o_items = align_up(o_views + sizeof(SicView) * (size_t)n, 256);
+    const size_t o_gsic = align_up(o_views + sizeof(SicView) * (size_t)n, 64);
+    const size_t o_gfirst = align_up(o_gsic + sizeof(GroupSic) * (size_t)n, 64);  // gather, then co-add
+    const size_t head = align_up(o_gfirst + 2 * sizeof(int32_t) * (size_t)(n + 1), 256);
+    const size_t o_items = head;
     std::vector<size_t> o_itemset(itemsets.size());
     {
         size_t o = o_items;
@@ -1114,7 +1119,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         return fail(NF_ERR_CAPACITY, "SIC group needs " + std::to_string(plan_total + work_total + slab_total) +
                                          " device bytes, buffer holds " + std::to_string(rt.max_buffer_bytes));
     if ((rc = w.plan.ensure(plan_total)) || (rc = w.work.ensure(work_total)) || (rc = w.pin_census.ensure(32)) ||
-        (rc = w.pin_plan.ensure(dplan ? sizeof(SicView) * (size_t)n : plan_total)))
+        (rc = w.pin_plan.ensure(dplan ? head : plan_total)))
         return rc;
     if (s0.mem == NF_MEM_HOST && (rc = w.slabs.ensure(slab_total))) return rc;
 
@@ -1198,6 +1203,23 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         v.W = s.nfwidth;
         v.R = s.nfradius;
     }
+    // The group-wide gather and co-add launches' view of each SIC.
+    GroupSic *gsic = reinterpret_cast<GroupSic *>(hp + o_gsic);
+    int32_t *gfirst = reinterpret_cast<int32_t *>(hp + o_gfirst), *cfirst = gfirst + (n + 1);
+    gfirst[0] = cfirst[0] = 0;
+    for (int i = 0; i < n; i++) {
+        GroupSic &G = gsic[i];
+        std::memset(&G, 0, sizeof(G));
+        G.pl = plans[i];
+        G.sl = slabs[i];
+        G.kick = kicks[i];
+        G.sources4 = wb + WL[i].o_src;
+        G.partial = wb + WL[i].o_part;
+        G.acc4 = accd[i];
+        G.out_scale = pc.out_scale;
+        gfirst[i + 1] = gfirst[i] + group_gather_waves(plans[i]);
+        cfirst[i + 1] = cfirst[i] + group_coadd_waves(plans[i]);
+    }
     for (size_t k = 0; k < itemsets.size(); k++)
         if (!itemsets[k].empty())
             std::memcpy(hp + o_itemset[k], itemsets[k].data(), sizeof(WorkItem) * itemsets[k].size());
@@ -1207,7 +1229,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     // plan buffer is free).  A device-planned SIC's image is its view only;
     // phase B and the work lists fill the rest on the planning stream.
     if (dplan) {
-        NF_HIP(hipMemcpyAsync(w.plan.p, hp, sizeof(SicView) * (size_t)n, hipMemcpyHostToDevice, w.pstream));
+        NF_HIP(hipMemcpyAsync(w.plan.p, hp, head, hipMemcpyHostToDevice, w.pstream));
         char *sc = static_cast<char *>(w.gscratch.p);
         int32_t *shared = reinterpret_cast<int32_t *>(sc + gs.o_shared);
         std::vector<GPlanOut> go(n);
@@ -1305,9 +1327,13 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         capture.on = true;
     }
     int e = 0;
-    for (int i = 0; i < n && !e; i++)
-        e = launch_gather(plans[i], slabs[i], path == NF_PATH_BATCHED,
-                          path == NF_PATH_BATCHED ? nullptr : wb + WL[i].o_sinks, wb + WL[i].o_src, stream);
+    const GroupSic *gsic_d = reinterpret_cast<const GroupSic *>(pd + o_gsic);
+    const int32_t *gfirst_d = reinterpret_cast<const int32_t *>(pd + o_gfirst), *cfirst_d = gfirst_d + (n + 1);
+    if (path == NF_PATH_BATCHED)
+        e = launch_gather_group(gsic_d, gfirst_d, n, gfirst[n], s0.cpd, stream);
+    else
+        for (int i = 0; i < n && !e; i++)
+            e = launch_gather(plans[i], slabs[i], false, wb + WL[i].o_sinks, wb + WL[i].o_src, stream);
     if (e) return fail(NF_ERR_HIP, std::string("gather launch: ") + hipGetErrorString((hipError_t)e));
     if (!use_graph) NF_HIP(hipEventRecord(w.ev[2], stream));
     std::vector<hipEvent_t> tev;
@@ -1353,11 +1379,11 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     }
     if (e) return fail(NF_ERR_HIP, std::string("direct kernel launch: ") + hipGetErrorString((hipError_t)e));
     if (!use_graph) NF_HIP(hipEventRecord(w.ev[3], stream));
-    for (int i = 0; i < n && !e; i++) {
-        if (path == NF_PATH_BATCHED)
-            e = launch_coadd_f64(plans[i], wb + WL[i].o_part, pc.out_scale, accd[i], kicks[i], stream);
-        else e = launch_coadd_plain(plans[i], wb + WL[i].o_part, accd[i], kicks[i], stream);
-    }
+    if (path == NF_PATH_BATCHED)
+        e = launch_coadd_group(gsic_d, cfirst_d, n, cfirst[n], stream);
+    else
+        for (int i = 0; i < n && !e; i++)
+            e = launch_coadd_plain(plans[i], wb + WL[i].o_part, accd[i], kicks[i], stream);
     if (e) return fail(NF_ERR_HIP, std::string("co-add launch: ") + hipGetErrorString((hipError_t)e));
     if (use_graph) {
         hipGraph_t graph = nullptr;
