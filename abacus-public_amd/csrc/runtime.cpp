@@ -241,6 +241,29 @@ struct Runtime {
 std::mutex g_rt_mutex;
 Runtime *g_rt = nullptr;
 
+// The runtime's streams (compute, upload, download, planning) live for the
+// whole process: made by the first nf_setup that needs them and reused by
+// every later one (under g_rt_mutex).  HIP maps a new stream onto one of its
+// hardware queues when the stream is made; streams destroyed and made again
+// by a second nf_setup can land two busy streams on one queue, where they
+// serialize (a second driver in one process measured 3% slower).
+struct StreamPool {
+    hipStream_t compute = nullptr, up = nullptr, down = nullptr, plan = nullptr;
+};
+StreamPool g_streams;
+
+hipError_t pooled_stream(hipStream_t *slot, hipStream_t *out) {
+    if (!*slot) {
+        const hipError_t e = hipStreamCreateWithFlags(slot, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            *slot = nullptr;
+            return e;
+        }
+    }
+    *out = *slot;
+    return hipSuccess;
+}
+
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Where one SIC's plan arrays sit inside a group's plan upload.
@@ -1713,19 +1736,16 @@ extern "C" int nf_setup(const nf_setup_params *p, int32_t *max_sink_blocks, int3
     if (max_sink_blocks) *max_sink_blocks = ms;
     if (max_source_blocks) *max_source_blocks = (int32_t)std::min<double>((double)ms * W, INT32_MAX / NF_BLOCK_SIZE);
     if (p->gpu_streams != 0 && p->gpu_streams != 1) return fail(NF_ERR_INVALID, "gpu_streams must be 0 or 1");
-    // Streams first (compute, upload, download, planning), then the workers.
-    auto destroy_streams = [&] {
-        for (hipStream_t st : {rt->shared, rt->pstream, rt->cache.up, rt->cache.down})
-            if (st) (void)hipStreamDestroy(st);
-        rt->shared = rt->pstream = rt->cache.up = rt->cache.down = nullptr;
-    };
+    // Streams first (compute, upload, download, planning: the process's
+    // pooled streams), then the workers.
+    auto destroy_streams = [&] { rt->shared = rt->pstream = rt->cache.up = rt->cache.down = nullptr; };
     {
         hipError_t e = hipSuccess;
-        if (p->gpu_streams == 1) e = hipStreamCreateWithFlags(&rt->shared, hipStreamNonBlocking);
+        if (p->gpu_streams == 1) e = pooled_stream(&g_streams.compute, &rt->shared);
         auto &C = rt->cache;
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&C.up, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&C.down, hipStreamNonBlocking);
-        if (e == hipSuccess && p->gpu_streams == 1) e = hipStreamCreateWithFlags(&rt->pstream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = pooled_stream(&g_streams.up, &C.up);
+        if (e == hipSuccess) e = pooled_stream(&g_streams.down, &C.down);
+        if (e == hipSuccess && p->gpu_streams == 1) e = pooled_stream(&g_streams.plan, &rt->pstream);
         for (int k = 0; k < Runtime::kStages && e == hipSuccess; k++)
             e = hipEventCreateWithFlags(&C.stage_done[k], hipEventDisableTiming);
         if (e != hipSuccess) {
@@ -1762,18 +1782,13 @@ extern "C" int nf_reset(void) {
     for (auto &w : g_rt->workers)
         if (w->th.joinable()) w->th.join();
     for (auto &w : g_rt->workers) destroy_worker(*w);
+    // The pooled streams stay for the next nf_setup, drained.
     for (hipStream_t st : {g_rt->shared, g_rt->pstream})
-        if (st) {
-            (void)hipStreamSynchronize(st);
-            (void)hipStreamDestroy(st);
-        }
+        if (st) (void)hipStreamSynchronize(st);
     {
         auto &C = g_rt->cache;
         for (hipStream_t st : {C.up, C.down})
-            if (st) {
-                (void)hipStreamSynchronize(st);
-                (void)hipStreamDestroy(st);
-            }
+            if (st) (void)hipStreamSynchronize(st);
         for (auto &sp : C.slots) {
             sp->buf.release();
             for (hipEvent_t ev : {sp->ready, sp->t0, sp->t1})
