@@ -14,6 +14,11 @@ namespace nf {
 // Record an error message for nf_last_error() on this thread; returns code.
 int fail(int code, const std::string &msg);
 
+// nf_count_interactions' sums (planner.cpp); with strict false a negative
+// count counts as zero.
+int count_interactions(int32_t cpd, int32_t R, int32_t j_low, int32_t j_high, const nf_cellinfo *const *ci,
+                       bool strict, uint64_t *out);
+
 // memcpy over nt threads, in 64-byte aligned chunks that cover every byte
 // (hostcopy.cpp); par_memcpy picks nt from the host's cores and the size.
 void par_memcpy_threads(void *dst, const void *src, size_t bytes, unsigned nt);

@@ -319,3 +319,67 @@ extern "C" int nf_split_slab(int32_t cpd, int32_t R, const nf_cellinfo *const *c
         return nf::fail(NF_ERR_CAPACITY, "nf_split_slab: sinks or sources of the last skewer overflow the maxima");
     return NF_OK;
 }
+
+namespace nf {
+
+// The separable cube sums behind nf_count_interactions; negative counts are
+// an error (strict) or count as zero.
+int count_interactions(int32_t cpd, int32_t R, int32_t j_low, int32_t j_high, const nf_cellinfo *const *ci,
+                       bool strict, uint64_t *out) {
+    const int W = 2 * R + 1, rows = (j_high - j_low) + 2 * R;
+    auto wrap = [&](int i) { return ((i % cpd) + cpd) % cpd; };
+    // m: particles of the W slabs in each cell of the source rows j_low-R ..
+    // j_high+R-1; then mz: m summed over z-R .. z+R.
+    std::vector<int64_t> m((size_t)rows * cpd), mz((size_t)rows * cpd);
+    for (int r = 0; r < rows; r++) {
+        const size_t row = (size_t)wrap(j_low - R + r) * cpd;
+        for (int z = 0; z < cpd; z++) {
+            int64_t n = 0;
+            for (int c = 0; c < W; c++) {
+                const int32_t k = ci[c][row + z].count;
+                if (k < 0) {
+                    if (strict) return fail(NF_ERR_INVALID, "nf_count_interactions: negative cell count");
+                    continue;
+                }
+                n += k;
+            }
+            m[(size_t)r * cpd + z] = n;
+        }
+        const int64_t *mr = &m[(size_t)r * cpd];
+        int64_t run = 0;
+        for (int dz = -R; dz <= R; dz++) run += mr[wrap(dz)];
+        for (int z = 0; z < cpd; z++) {
+            mz[(size_t)r * cpd + z] = run;
+            run += mr[wrap(z + R + 1)] - mr[wrap(z - R)];
+        }
+    }
+    uint64_t total = 0;
+    for (int j = j_low; j < j_high; j++) {
+        const size_t row = (size_t)j * cpd;
+        const int r0 = j - j_low;  // rows r0 .. r0 + 2R hold j-R .. j+R
+        for (int z = 0; z < cpd; z++) {
+            const int32_t k = ci[R][row + z].count;
+            if (k <= 0) continue;
+            int64_t cube = 0;
+            for (int dy = 0; dy < W; dy++) cube += mz[(size_t)(r0 + dy) * cpd + z];
+            total += (uint64_t)k * (uint64_t)cube;
+        }
+    }
+    *out = total;
+    return NF_OK;
+}
+
+}  // namespace nf
+
+extern "C" int nf_count_interactions(int32_t cpd, int32_t nfradius, int32_t j_low, int32_t j_high,
+                                     const nf_cellinfo *const *cellinfo, uint64_t *out) {
+    if (!cellinfo || !out) return nf::fail(NF_ERR_INVALID, "nf_count_interactions: null argument");
+    if (cpd <= 0 || (cpd % 2) == 0) return nf::fail(NF_ERR_INVALID, "nf_count_interactions: cpd must be odd and positive");
+    if (nfradius < 0 || nfradius > NF_MAX_RADIUS || 2 * nfradius + 1 > cpd)
+        return nf::fail(NF_ERR_INVALID, "nf_count_interactions: nfradius out of range");
+    if (j_low < 0 || j_high > cpd || j_low >= j_high)
+        return nf::fail(NF_ERR_INVALID, "nf_count_interactions: bad sink row range");
+    for (int c = 0; c < 2 * nfradius + 1; c++)
+        if (!cellinfo[c]) return nf::fail(NF_ERR_INVALID, "nf_count_interactions: missing cellinfo slab");
+    return nf::count_interactions(cpd, nfradius, j_low, j_high, cellinfo, true, out);
+}

@@ -94,6 +94,7 @@ struct Task {
         GPlanTotals tot{};        // phase A's sizes, read back with the group's results
         uint64_t chunks_bound = 0, items_bound = 0;  // bounds the plan is laid out for
         uint64_t occ[4] = {};     // sinks in pencils of <64, 64-255, 256-1023, >=1024 sinks
+        uint64_t interactions = 0;  // DirectTotal from the cellinfo counts (nf_count_interactions)
     } dp;
     std::atomic<int> state{0};  // 0 queued/running, 1 done, <0 failed
     std::string error;
@@ -482,9 +483,7 @@ size_t task_bytes(const nf_sic &s, int path, int32_t n_chunks) {
 
 // A device-planned SIC before phase A: upper bounds of its sizes from its
 // slab sizes (every pencil adds at most one partly filled block; a source
-// row that wraps round the box appears twice), and an estimate of its pair
-// interactions for a uniform box, (2R+1)^2 n_sink n_window / cpd^2.  For
-// grouping decisions only.
+// row that wraps round the box appears twice).
 // Sets s's block counts to those bounds; *chunks and *items get the bounds
 // of its co-add chunks and of its work items (at most one per padded sink
 // block plus one tail per pencil, for every tile).  The device planner's
@@ -523,16 +522,10 @@ void dp_occupancy(const Task &t, uint64_t occ[4]) {
         }
     }
 }
-uint64_t dp_interactions_estimate(const nf_sic &s) {
-    double window = 0.0;
-    for (int c = 0; c < s.nfwidth; c++) window += (double)s.Nslab[c];
-    const double W = s.nfwidth, cpd = s.cpd;
-    return (uint64_t)(W * W * (double)s.Nslab[s.nfradius] * window / (cpd * cpd) * s.j_width / cpd);
-}
 size_t queued_bytes(const Task &t, int path) {
     return t.dp.on ? dp_task_bytes(t.sic, path) : task_bytes(t.sic, path, count_coadd_chunks(t.sic));
 }
-uint64_t queued_interactions(const Task &t) { return t.dp.on ? dp_interactions_estimate(t.sic) : t.sic.DirectTotal; }
+uint64_t queued_interactions(const Task &t) { return t.dp.on ? t.dp.interactions : t.sic.DirectTotal; }
 
 // The 1/8-octave bucket of an item's cost (sinks x sources), longest first:
 // bucket 0 holds the most expensive items, kItemBuckets - 1 the cheapest.
@@ -718,7 +711,7 @@ std::string occupancy_key(const std::vector<Task *> &g) {
     double tot = 0;
     uint64_t di = 0;
     for (Task *t : g) {
-        di += queued_interactions(*t);  // device-planned: an estimate, before any plan exists
+        di += queued_interactions(*t);  // device-planned: counted from the cellinfo, before any plan exists
         if (t->dp.on) {
             for (int b = 0; b < 4; b++) {
                 h[b] += (double)t->dp.occ[b];
@@ -928,11 +921,12 @@ int plan_phase_a(Runtime &rt, Worker &w, std::vector<Task *> &g, std::vector<Run
     char *sc = static_cast<char *>(w.gscratch.p);
     int32_t *shared = reinterpret_cast<int32_t *>(sc + gs.o_shared);
     in.assign(n, GPlanIn{});
-    // Heavy register tiles are split against the group's estimated work (the
-    // three planning phases all use this one threshold, so the item counts of
-    // phase A are those the later phases write).
+    // Heavy register tiles are split against the group's work, counted from
+    // the cellinfo at submission (the three planning phases all use this one
+    // threshold, so the item counts of phase A are those the later phases
+    // write).
     uint64_t group_est = 0;
-    for (Task *t : g) group_est += dp_interactions_estimate(t->sic);
+    for (Task *t : g) group_est += t->dp.interactions;
     for (size_t i = 0; i < n; i++) {
         Task &t = *g[i];
         nf_sic &s = t.sic;
@@ -1565,6 +1559,8 @@ bool compatible(const nf_sic &a, const nf_sic &b) {
     return a.softening == b.softening && a.eps == b.eps && a.b2 == b.b2 && a.mem == b.mem && a.cpd == b.cpd;
 }
 
+constexpr int kGroupFillWaitUs = 1000;
+
 void worker_main(Runtime *rt, Worker *w) {
     for (;;) {
         std::vector<Task *> g;
@@ -1573,6 +1569,22 @@ void worker_main(Runtime *rt, Worker *w) {
             if (rt->queue.empty()) rt->warm = false;
             rt->qcv.wait(lk, [&] { return rt->stopping || !rt->queue.empty(); });
             if (rt->queue.empty()) return;  // stopping and drained
+            // With work in flight, wait a little for a whole group's worth of
+            // SICs: a launch of one or two small SICs runs at ~2/3 of the
+            // kernel's rate (its longest items set its length), and the GPU
+            // has a group to run meanwhile.
+            if (rt->params.path == NF_PATH_BATCHED && rt->inflight > 0) {
+                const auto deadline = std::chrono::steady_clock::now() + std::chrono::microseconds(kGroupFillWaitUs);
+                auto full = [&] {
+                    if (rt->stopping || rt->queue.empty() || (int)rt->queue.size() >= rt->max_group) return true;
+                    uint64_t work = 0;
+                    for (const Task *t : rt->queue) work += queued_interactions(*t);
+                    return work >= rt->group_interactions;
+                };
+                while (!full() && rt->qcv.wait_until(lk, deadline) != std::cv_status::timeout) {
+                }
+                if (rt->queue.empty()) continue;  // another worker took them
+            }
             g.push_back(rt->queue.front());
             rt->queue.pop_front();
             // Device-planned SICs are grouped with each other only, on
@@ -1877,6 +1889,10 @@ extern "C" int nf_submit_slab(const nf_slab_task *p, int32_t blocking, nf_task *
     s.SinkAccSlab = p->SinkAccSlab;
     t->dp.on = true;
     for (int c = 0; c < W; c++) t->dp.cellinfo[c] = p->cellinfo[c];
+    // The SIC's exact pair count sizes its launch group and the heavy-tile
+    // split before the device plans it (a bad count fails the task in phase A).
+    if (count_interactions(p->cpd, R, p->j_low, p->j_high, p->cellinfo, false, &t->dp.interactions) != NF_OK)
+        t->dp.interactions = 0;
     // The sink rows run from cell (j_low, 0) to the end of cell (j_high-1,
     // cpd-1); phase A checks both cells against the slab.
     const nf_cellinfo &first = p->cellinfo[R][(size_t)p->j_low * p->cpd];

@@ -139,6 +139,7 @@ def load_library(path=LIB_PATH):
     lib.nf_plan_sic_device.argtypes = lib.nf_plan_sic.argtypes
     lib.nf_split_slab.argtypes = [I32, I32, C.POINTER(C.c_void_p), C.POINTER(U64), I32, I32, I32,
                                   C.POINTER(I32), C.POINTER(I32)]
+    lib.nf_count_interactions.argtypes = [I32, I32, I32, I32, C.POINTER(C.c_void_p), C.POINTER(U64)]
     lib.nf_submit_slab.argtypes = [C.POINTER(nf_slab_task), I32, C.POINTER(C.c_void_p)]
     lib.nf_device_alloc.argtypes = [U64, C.POINTER(C.c_void_p)]
     lib.nf_device_free.argtypes = [C.c_void_p]
@@ -522,3 +523,15 @@ class NearFieldDriver:
 
     def __exit__(self, *a):
         self.close()
+
+
+def count_interactions(box, slab, j_low=0, j_high=None, nfradius=2):
+    """nf_count_interactions: the pair interactions (DirectTotal) of the SIC of
+    ``slab``, rows [j_low, j_high), from the box's cellinfo counts alone."""
+    if j_high is None:
+        j_high = box.cpd
+    W = 2 * nfradius + 1
+    ci = (C.c_void_p * W)(*[box.slab(slab - nfradius + c).cellinfo.ctypes.data for c in range(W)])
+    out = C.c_uint64()
+    check(lib().nf_count_interactions(box.cpd, nfradius, j_low, j_high, ci, C.byref(out)), "nf_count_interactions")
+    return int(out.value)

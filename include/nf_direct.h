@@ -379,6 +379,17 @@ int nf_split_slab(int32_t cpd, int32_t nfradius, const nf_cellinfo *const *celli
                   int32_t nbuffers, int32_t max_sink_blocks, int32_t max_source_blocks, int32_t *split,
                   int32_t *nsplit);
 
+/* The pair interactions of a SIC (its DirectTotal: self pairs included, no
+ * padding) from the cellinfo counts alone, as CalcDI counts a slab's
+ * (DirectBench.cpp:111-125), for the sink rows [j_low, j_high): each sink
+ * cell's count times the particles in its (2 nfradius + 1)^3 cube of cells,
+ * with the periodic wrap.  The cube sums are separable (over the slabs, then
+ * z, then y), so a CPD 255 slab takes about a millisecond on one core.
+ * nf_submit_slab uses it to size launch groups before any plan exists.
+ * cellinfo[c] as for nf_plan_sic; NF_ERR_INVALID for a negative count. */
+int nf_count_interactions(int32_t cpd, int32_t nfradius, int32_t j_low, int32_t j_high,
+                          const nf_cellinfo *const *cellinfo, uint64_t *out);
+
 /* nf_plan_sic computed by the device planner nf_submit_slab uses (same
  * arguments, same output; cellinfo in host memory; nslab must be given, as
  * the device planner checks every cell against its slab).  Synchronous, on

@@ -1,6 +1,6 @@
 """pytest plugin (loaded with -p san_plugin) that points the CPU tests at
 sanitized builds: the planner entry points (nf_sic_arena_bytes, nf_plan_sic,
-nf_split_slab) from a host-only build of planner.cpp, and the oracle
+nf_split_slab, nf_count_interactions) from a host-only build of planner.cpp, and the oracle
 libraries, all compiled with -fsanitize=address,undefined into the folder
 NF_SANITIZED_LIBDIR names.  The process runs with libasan preloaded."""
 import ctypes as C
@@ -27,6 +27,7 @@ def _planner_library(path):
                                 C.POINTER(nfm.nf_sic)]
     lib.nf_split_slab.argtypes = [I32, I32, C.POINTER(C.c_void_p), C.POINTER(U64), I32, I32, I32,
                                   C.POINTER(I32), C.POINTER(I32)]
+    lib.nf_count_interactions.argtypes = [I32, I32, I32, I32, C.POINTER(C.c_void_p), C.POINTER(U64)]
     return lib
 
 

@@ -117,6 +117,34 @@ def test_direct_total_sums_over_splits():
         assert sum(q["DirectTotal"] for q in parts) == whole["DirectTotal"]
 
 
+@pytest.mark.parametrize("name", sorted(BOXES))
+def test_count_interactions_is_calc_di_and_direct_total(name):
+    """nf_count_interactions (separable cube sums over the cellinfo counts)
+    equals the oracle's brute-force CalcDI over whole slabs, and the
+    planner's DirectTotal over row ranges, including ranges that wrap the
+    source rows round the box and single rows."""
+    box = BOXES[name]()
+    cpd = box.cpd
+    for x in (0, cpd // 2, cpd - 1):
+        assert nfm.count_interactions(box, x) == po.calc_di(box, R, x), (name, x)
+        for jl, jh in [(0, 1), (0, 3), (cpd - 2, cpd), (1, cpd - 1), (cpd // 2, cpd // 2 + 1)]:
+            prod, _ = both(box, x, jl, jh)
+            assert nfm.count_interactions(box, x, jl, jh) == int(prod["DirectTotal"]), (name, x, jl, jh)
+
+
+def test_count_interactions_radius_one_and_errors():
+    box = BOXES["uniform9"]()
+    for x in (0, 5):
+        assert nfm.count_interactions(box, x, nfradius=1) == po.calc_di(box, 1, x)
+    bad = synth.Box(box.cpd, [synth.Slab(s.pos, s.count) for s in box.slabs], "negative count")
+    bad.slabs[0].cellinfo["count"][7] = -1
+    with pytest.raises(nfm.NFError, match="negative"):
+        nfm.count_interactions(bad, 0)
+    for jl, jh in [(0, 0), (-1, 2), (3, box.cpd + 1)]:
+        with pytest.raises(nfm.NFError, match="row range"):
+            nfm.count_interactions(box, 0, jl, jh)
+
+
 def test_coadd_set_then_add_equals_sum():
     """The k-ordered set/add co-add (PencilPlan.cc:140-207) equals zero-then-add:
     with integer-valued partials every summation order is exact."""
