@@ -228,6 +228,10 @@ def run_resident(args, box, cpd, paths, steps, warmup, keep_acc=False):
                 sics = None  # each step plans its own
 
             def step():
+                # One stream session per step: the slabs and their cellinfo do
+                # not change within a step, so each cellinfo slab goes up once
+                # (device planner) instead of once per launch group.
+                drv.stream_begin()
                 if args.planner == "device":
                     tasks = [drv.submit_slab(box, x, pos_ptrs=ptrs_of[x], acc_ptr=acc_d[x].ptr,
                                              mem=nfm.NF_MEM_DEVICE) for x in range(cpd)]
@@ -241,6 +245,7 @@ def run_resident(args, box, cpd, paths, steps, warmup, keep_acc=False):
                     t.wait()
                     st.append(t.stats())
                     t.release()
+                drv.stream_end()
                 return st
 
             for i in range(warmup):

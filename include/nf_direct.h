@@ -320,10 +320,11 @@ int nf_autotune_table(char *buf, size_t len);
 /* ---- host-slab streaming ---------------------------------------------
  * No counterpart in the reference.  Between nf_stream_begin and
  * nf_stream_end the caller promises that the host PosXYZ slabs named by
- * submitted SICs do not change (the reference's pipeline keeps a PosXYZ slab
- * fixed while SICs that read it are pending).  Inside such a session the
- * module keeps recently used host slabs in HBM keyed by address, so
- * consecutive slab SICs upload each slab once; uploads and acceleration
+ * submitted SICs, and the cellinfo slabs named by nf_submit_slab tasks, do
+ * not change (the reference's pipeline keeps a slab fixed while SICs that
+ * read it are pending; one session per step is the usual bracket).  Inside
+ * such a session the module keeps recently used host slabs in HBM keyed by
+ * address, so consecutive slab SICs upload each slab once; uploads and acceleration
  * copy-backs run on a copy stream through double-buffered pinned staging
  * and overlap other groups' kernels.  Outside a session every group uploads
  * its slabs afresh.  nf_stream_end waits for no task; slabs still in use are
