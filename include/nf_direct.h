@@ -385,7 +385,7 @@ int nf_split_slab(int32_t cpd, int32_t nfradius, const nf_cellinfo *const *celli
  * (DirectBench.cpp:111-125), for the sink rows [j_low, j_high): each sink
  * cell's count times the particles in its (2 nfradius + 1)^3 cube of cells,
  * with the periodic wrap.  The cube sums are separable (over the slabs, then
- * z, then y), so a CPD 255 slab takes about a millisecond on one core.
+ * z, then y), so a whole CPD 255 slab takes about 3 ms on one core.
  * nf_submit_slab uses it to size launch groups before any plan exists.
  * cellinfo[c] as for nf_plan_sic; NF_ERR_INVALID for a negative count. */
 int nf_count_interactions(int32_t cpd, int32_t nfradius, int32_t j_low, int32_t j_high,
