@@ -377,13 +377,17 @@ __global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSl
     gather_pencil_binned(pl, sl, out, box, cpdf, invcpd, s, threadIdx.x & (kWave - 1));
 }
 
-// The SIC of a group-wide launch that wave w belongs to (first[] ascending,
-// first[n] the total; a group holds a few SICs).
+// The SIC of a group-wide launch that wave w belongs to: the last i with
+// first[i] <= w (first[] ascending, first[n] the total), by bisection.
 __device__ __forceinline__ int group_sic_of(const int32_t *first, int n, int w) {
     const NF_CONST int32_t *f = cptr<int32_t>(first);
-    int i = 0;
-    while (i < n - 1 && w >= f[i + 1]) i++;
-    return i;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (f[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Every SIC's source gather of a launch group in one launch: a group of
