@@ -219,8 +219,8 @@ __device__ void plan_b1(const GPlanIn &in, const GPlanOut &out, int bid) {
         // The work-list histogram is built in LDS, then added to the group's
         // with one atomic per workgroup and bucket: nearly every pencil of a
         // uniform slab falls in the same bucket.
-        __shared__ int32_t lh[3 * kPlanBuckets];
-        for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT) lh[i] = 0;
+        __shared__ int32_t lh[kPlanLists * kPlanBuckets];
+        for (int i = threadIdx.x; i < kPlanLists * kPlanBuckets; i += kT) lh[i] = 0;
         __syncthreads();
         const int s = bid * kT + threadIdx.x;
         int nb = 0, nc = 0;
@@ -256,7 +256,7 @@ __device__ void plan_b1(const GPlanIn &in, const GPlanOut &out, int bid) {
                     int32_t *h = lh + ti * kPlanBuckets;
                     if (sp.full) atomicAdd(h + cost_bucket((uint64_t)sp.tile * nsrc), sp.full);
                     if (sp.n2) atomicAdd(h + cost_bucket((uint64_t)kB * nsrc), sp.n2);
-                    if (sp.tail) atomicAdd(h + cost_bucket((uint64_t)sp.s * nsrc), 1);
+                    if (sp.tail) atomicAdd(h + 3 * kPlanBuckets + cost_bucket((uint64_t)sp.s * nsrc), 1);
                 }
             }
         }
@@ -267,7 +267,7 @@ __device__ void plan_b1(const GPlanIn &in, const GPlanOut &out, int bid) {
             out.part[bid] = tb;
             out.part[GP + bid] = tc;
         }
-        for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT)
+        for (int i = threadIdx.x; i < kPlanLists * kPlanBuckets; i += kT)
             if (lh[i]) atomicAdd(out.hist + i, lh[i]);
     } else {
         const int g = bid - GP;
@@ -305,11 +305,14 @@ __global__ __launch_bounds__(kT) void k_plan_b1(GPlanIn in, GPlanOut out) { plan
 // bucket).  Each item run gets its place in two steps: an offset within the
 // workgroup's share of its bucket (LDS atomics), then the share's base (one
 // global atomic per workgroup and bucket).  boff: the buckets' offsets in
-// each list.  Every thread of the workgroup must call it.
-__device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_t (*boff)[kPlanBuckets], int bid) {
-    __shared__ int32_t lc[3 * kPlanBuckets];
+// each list's run of tiles (0-2) and of short tails (3-5); mt: the runs'
+// lengths, M then T per list.  The two runs merge by interleave_pos.  Every
+// thread of the workgroup must call it.
+__device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_t (*boff)[kPlanBuckets],
+                            const int32_t *mt, int bid) {
+    __shared__ int32_t lc[kPlanLists * kPlanBuckets];
     const int P = in.jw * in.cpd;
-    for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT) lc[i] = 0;
+    for (int i = threadIdx.x; i < kPlanLists * kPlanBuckets; i += kT) lc[i] = 0;
     __syncthreads();
     const int t = bid * kT + threadIdx.x;
     const int s = t < P ? (t % in.jw) * in.cpd + t / in.jw : 0;
@@ -325,10 +328,10 @@ __device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_
         int32_t *h = lc + ti * kPlanBuckets;
         lo[ti][0] = on && sp.full ? atomicAdd(h + bk[ti][0], sp.full) : 0;
         lo[ti][1] = on && sp.n2 ? atomicAdd(h + bk[ti][1], sp.n2) : 0;
-        lo[ti][2] = on && sp.tail ? atomicAdd(h + bk[ti][2], 1) : 0;
+        lo[ti][2] = on && sp.tail ? atomicAdd(h + 3 * kPlanBuckets + bk[ti][2], 1) : 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 3 * kPlanBuckets; i += kT)
+    for (int i = threadIdx.x; i < kPlanLists * kPlanBuckets; i += kT)
         if (lc[i]) lc[i] = atomicAdd(out.cursor + i, lc[i]);
     __syncthreads();
     if (n == 0) return;
@@ -337,17 +340,23 @@ __device__ void place_items(const GPlanIn &in, const GPlanOut &out, const int32_
         WorkItem *items = out.items[ti];
         if (!items) continue;
         const ItemSplit sp(n, 1 << ti, nsrc, in.split_cost[ti]);
-        const int32_t *base = lc + ti * kPlanBuckets;
+        const int32_t *base = lc + ti * kPlanBuckets, *tbase = lc + (3 + ti) * kPlanBuckets;
+        const int64_t M = mt[ti], T = mt[3 + ti];
         if (sp.full) {
             const int p = boff[ti][bk[ti][0]] + base[bk[ti][0]] + lo[ti][0];
-            for (int i = 0; i < sp.full; i++) items[p + i] = WorkItem{s, i * sp.tile, sp.tile, 0 + sic4};
+            for (int i = 0; i < sp.full; i++)
+                items[interleave_pos(false, p + i, M, T)] = WorkItem{s, i * sp.tile, sp.tile, 0 + sic4};
         }
         const int f = sp.full * sp.tile;
         if (sp.n2) {
             const int p = boff[ti][bk[ti][1]] + base[bk[ti][1]] + lo[ti][1];
-            for (int i = 0; i < sp.n2; i++) items[p + i] = WorkItem{s, f + i * kB, i < sp.q ? kB : sp.s, 2 + sic4};
+            for (int i = 0; i < sp.n2; i++)
+                items[interleave_pos(false, p + i, M, T)] = WorkItem{s, f + i * kB, i < sp.q ? kB : sp.s, 2 + sic4};
         }
-        if (sp.tail) items[boff[ti][bk[ti][2]] + base[bk[ti][2]] + lo[ti][2]] = WorkItem{s, f + sp.q * kB, sp.s, 1 + sic4};
+        if (sp.tail) {
+            const int p = boff[3 + ti][bk[ti][2]] + tbase[bk[ti][2]] + lo[ti][2];
+            items[interleave_pos(true, p, M, T)] = WorkItem{s, f + sp.q * kB, sp.s, 1 + sic4};
+        }
     }
 }
 
@@ -357,17 +366,20 @@ __device__ void plan_b2(const GPlanIn &in, const GPlanOut &out, int bid) {
     const int P = in.jw * in.cpd, Q = (in.jw + in.W - 1) * in.cpd;
     const int GP = ngroups(P);
     if (bid < GP) {
-        // Bucket offsets of the wanted work lists: exclusive scan of the
-        // histogram in LDS (2 buckets per thread).
-        __shared__ int32_t boff[3][kPlanBuckets];
-        for (int ti = 0; ti < 3; ti++) {
-            if (!out.items[ti]) continue;
-            const int32_t *h = out.hist + ti * kPlanBuckets;
+        // Bucket offsets of the wanted work lists' runs of tiles and of
+        // short tails: exclusive scans of the histograms in LDS (2 buckets
+        // per thread), and the runs' lengths.
+        __shared__ int32_t boff[kPlanLists][kPlanBuckets];
+        __shared__ int32_t mt[kPlanLists];
+        for (int li = 0; li < kPlanLists; li++) {
+            if (!out.items[li % 3]) continue;
+            const int32_t *h = out.hist + li * kPlanBuckets;
             const int a = h[2 * threadIdx.x], b = h[2 * threadIdx.x + 1];
             int total;
             const int e = block_excl_scan(a + b, &total);
-            boff[ti][2 * threadIdx.x] = e;
-            boff[ti][2 * threadIdx.x + 1] = e + a;
+            boff[li][2 * threadIdx.x] = e;
+            boff[li][2 * threadIdx.x + 1] = e + a;
+            if (threadIdx.x == 0) mt[li] = total;
         }
         const int s = bid * kT + threadIdx.x;
         const int n = s < P ? out.sink_count[s] : 0;
@@ -375,7 +387,7 @@ __device__ void plan_b2(const GPlanIn &in, const GPlanOut &out, int bid) {
         int tb, tc;
         const int b0 = block_sum_prefix(out.part, bid) + block_excl_scan(nblocks(n), &tb);
         const int c0 = block_sum_prefix(out.part + GP, bid) + block_excl_scan(nblocks(n0), &tc);
-        place_items(in, out, boff, bid);
+        place_items(in, out, boff, mt, bid);
         if (s >= P) return;
         const int j = s / in.cpd, k = s % in.cpd;
         out.sink_start[s] = b0 * kB;

@@ -324,7 +324,7 @@ GPlanOut plan_out(char *base, const PlanLayout &L, int sic, int32_t *shared, int
     o.src_parent = reinterpret_cast<int32_t *>(base + L.o_src_parent);
     o.chunks = reinterpret_cast<int32_t *>(base + L.o_chunks);
     o.hist = shared;
-    o.cursor = shared ? shared + 3 * kPlanBuckets : nullptr;
+    o.cursor = shared ? shared + kPlanLists * kPlanBuckets : nullptr;
     o.part = part;
     o.sic = sic;
     return o;
@@ -534,6 +534,9 @@ uint16_t cost_bucket(uint64_t cost) {
     const uint64_t b = cost ? (uint64_t)std::floor(8.0 * std::log2((double)cost)) : 0;
     return (uint16_t)(kItemBuckets - 1 - std::min<uint64_t>(b, kItemBuckets - 1));
 }
+// Counting-sort key of an item: its bucket among the tiles, or kItemBuckets
+// plus its bucket among the short tails.
+int item_key(const WorkItem &w, uint16_t bucket) { return (int)bucket + ((w.kind_sic & 3) == 1 ? kItemBuckets : 0); }
 
 // The batched work list of one SIC, appended in sink-pencil order z then y
 // (pencils adjacent in y share 2R of their 2R+1 source pencils), with each
@@ -1038,7 +1041,8 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     // Work list over the whole group, longest items first: one list per
     // distinct sinks-per-lane value to be launched (several while tuning).
     // Items within 1/8 octave of cost count as equal and are kept in sink
-    // pencil order (SIC, z, y): a stable counting sort by bucket.
+    // pencil order (SIC, z, y): a stable counting sort by bucket, of the
+    // tiles and of the short tails apart, merged by interleave_pos.
     uint64_t group_total = 0;
     for (int i = 0; i < n; i++) group_total += g[i]->sic.DirectTotal;
     auto make_items = [&](int S) {
@@ -1046,11 +1050,16 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         std::vector<uint16_t> bucket;
         const uint64_t max_tile_cost = tile_split_cost(s0.softening, S, group_total);
         for (int i = 0; i < n; i++) build_items(g[i]->sic, S, i, max_tile_cost, raw, bucket);
-        std::vector<size_t> start(kItemBuckets + 1, 0);
-        for (uint16_t b : bucket) start[b + 1]++;
-        for (int b = 0; b < kItemBuckets; b++) start[b + 1] += start[b];
+        std::vector<size_t> start(2 * kItemBuckets + 1, 0);
+        for (size_t i = 0; i < raw.size(); i++) start[item_key(raw[i], bucket[i]) + 1]++;
+        for (int b = 0; b < 2 * kItemBuckets; b++) start[b + 1] += start[b];
         out.resize(raw.size());
-        for (size_t i = 0; i < raw.size(); i++) out[start[bucket[i]]++] = raw[i];
+        const int64_t M = (int64_t)start[kItemBuckets], T = (int64_t)raw.size() - M;
+        for (size_t i = 0; i < raw.size(); i++) {
+            const int key = item_key(raw[i], bucket[i]);
+            const int64_t r = (int64_t)start[key]++;
+            out[key < kItemBuckets ? interleave_pos(false, r, M, T) : interleave_pos(true, r - M, M, T)] = raw[i];
+        }
         return out;
     };
     std::vector<int> Svals;
@@ -2176,7 +2185,7 @@ char *g_noalloc_dev[kNoallocMaxDevices];  // per device: the address of `scratch
 // sort's cursors, static so that a call allocates no host memory either.
 constexpr size_t kNoallocHostBytes = 32u << 20;
 alignas(256) char g_noalloc_host[kNoallocHostBytes];
-uint32_t g_noalloc_cursor[kItemBuckets + 1];
+uint32_t g_noalloc_cursor[2 * kItemBuckets + 1];
 
 // Calls share the scratch of their GPU and the host image: one at a time.
 std::mutex g_noalloc_m;
@@ -2253,19 +2262,25 @@ extern "C" int nf_execute_sic_noalloc(const nf_sic *sic, void *stream) {
     const DevPlan pl = dev_plan(s, L.plan, d);
 
     // The work list, longest items first (items within 1/8 octave of cost
-    // keep sink-pencil order): a counting sort by cost bucket in two passes.
+    // keep sink-pencil order), short tails spread through the tiles: a
+    // counting sort by cost bucket in two passes, as the batched path's.
     constexpr int S = 2;  // every tile shape gives bitwise the same partials
     const uint64_t split = tile_split_cost(s.softening, S, s.DirectTotal);
-    std::fill(g_noalloc_cursor, g_noalloc_cursor + kItemBuckets + 1, 0u);
+    std::fill(g_noalloc_cursor, g_noalloc_cursor + 2 * kItemBuckets + 1, 0u);
     size_t nitems = 0;
-    for_each_item(s, S, 0, split, [&](const WorkItem &, uint16_t b) {
-        g_noalloc_cursor[b + 1]++;
+    for_each_item(s, S, 0, split, [&](const WorkItem &w, uint16_t b) {
+        g_noalloc_cursor[item_key(w, b) + 1]++;
         nitems++;
     });
     if (nitems > L.items_max) return fail(NF_ERR_STATE, "work list longer than its bound");
-    for (int b = 0; b < kItemBuckets; b++) g_noalloc_cursor[b + 1] += g_noalloc_cursor[b];
+    for (int b = 0; b < 2 * kItemBuckets; b++) g_noalloc_cursor[b + 1] += g_noalloc_cursor[b];
+    const int64_t M = g_noalloc_cursor[kItemBuckets], T = (int64_t)nitems - M;
     WorkItem *items = reinterpret_cast<WorkItem *>(h + L.o_items);
-    for_each_item(s, S, 0, split, [&](const WorkItem &w, uint16_t b) { items[g_noalloc_cursor[b]++] = w; });
+    for_each_item(s, S, 0, split, [&](const WorkItem &w, uint16_t b) {
+        const int key = item_key(w, b);
+        const int64_t r = g_noalloc_cursor[key]++;
+        items[key < kItemBuckets ? interleave_pos(false, r, M, T) : interleave_pos(true, r - M, M, T)] = w;
+    });
 
     DevSlabs sl{};
     for (int c = 0; c < s.nfwidth; c++) {

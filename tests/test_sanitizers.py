@@ -5,7 +5,9 @@ compiled with -fsanitize=address,undefined (undefined behaviour is fatal),
 then the planner and oracle CPU tests run against those builds in a child
 process with libasan preloaded; and hostcopy.cpp's threaded copy (the copy of
 pageable slabs into pinned staging) checked byte for byte over sizes and
-thread counts.  A sanitizer report fails the test.
+thread counts; and the work lists' merge of tiles and short tails
+(interleave_pos) checked to be an even permutation.  A sanitizer report
+fails the test.
 CPU only: nothing here touches a GPU."""
 import os
 import shutil
@@ -85,3 +87,20 @@ def test_threaded_host_copy_covers_every_byte(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600,
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="halt_on_error=1"))
     assert r.returncode == 0 and "copied exactly" in r.stdout, (r.stdout[-2000:], r.stderr[-2000:])
+
+
+def test_work_list_interleave_is_an_even_permutation(tmp_path):
+    """interleave_pos, which merges a work list's run of tiles with its run of
+    short tails (host lists, the no-allocation entry point's list and the
+    device planner's), places every item exactly once, keeps each run in its
+    cost order and spreads the rarer kind evenly, for every M, T up to 300
+    and launch-group sizes."""
+    if not shutil.which("g++"):
+        pytest.skip("g++ not found")
+    csrc = os.path.join(ROOT, "abacus-public_amd", "csrc")
+    exe = str(tmp_path / "interleave_check")
+    _run(["g++", "-std=c++17", "-O1", *SAN, "-I", os.path.join(ROOT, "include"), "-I", csrc,
+          os.path.join(ROOT, "tests", "sanitize", "interleave_check.cpp"), "-o", exe])
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, UBSAN_OPTIONS="halt_on_error=1"))
+    assert r.returncode == 0 and "even permutation" in r.stdout, (r.stdout[-2000:], r.stderr[-2000:])
