@@ -747,25 +747,28 @@ __device__ __forceinline__ void census_add(Census &c, int mode, uint64_t n) {
 
 // One source tile (cnt <= 64 sources) against the S sinks
 // of every lane; sources are wave-uniform loads (scalar cache).  The force
-// terms of each batch of 8 sources are summed on their own before they join
-// the tile sum: a sum of 8 batch sums of 8 terms carries about 1.6x less
-// FP32 rounding than one running sum of 64 terms, for one add per component
-// per batch.  Sink slots u < S/2 take pair form MLO, the others MHI (S = 1:
-// MHI); every form gives the same value for the pairs its box test admits,
-// so the split changes which code runs, not the result.  The last batch
-// takes only the cnt % 8 real sources: the padding sources after them give
-// exact zeros, so leaving them out changes no sum.
+// terms of each batch of kSrcBatch sources are summed on their own before
+// they join the tile sum, for one add per component per batch.  On random
+// terms a sum of 4 batch sums of 16 terms carries 1.7-2.3x less FP32
+// rounding than one running sum of 64 terms, and 1.0-1.1x the rounding of 8
+// batch sums of 8 terms, which cost 1% more kernel time.  Sink slots
+// u < S/2 take pair form MLO, the others MHI (S = 1: MHI); every form gives
+// the same value for the pairs its box test admits, so the split changes
+// which code runs, not the result.  The last batch takes only the
+// cnt % kSrcBatch real sources: the padding sources after them give exact
+// zeros, so leaving them out changes no sum.
+constexpr int kSrcBatch = 16;
 template <int SOFT, int MLO, int MHI, int S>
 __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt, const float *kx, const float *ky,
                                           const float *kz, const PairConst &pc, float *tx, float *ty, float *tz,
                                           float *tw) {
-    const int full8 = cnt & ~7;
-    for (int i = 0; i < full8; i += 8) {
+    const int fullb = cnt & ~(kSrcBatch - 1);
+    for (int i = 0; i < fullb; i += kSrcBatch) {
         float bx[S], by[S], bz[S];
 #pragma unroll
         for (int u = 0; u < S; u++) bx[u] = by[u] = bz[u] = 0.f;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
+        for (int q = 0; q < kSrcBatch; q++) {
             const float4 sv = sp[i + q];
 #pragma unroll
             for (int u = 0; u < S / 2; u++)
@@ -781,11 +784,11 @@ __device__ __forceinline__ void full_tile(const NF_CONST float4 *sp, int cnt, co
             tz[u] += bz[u];
         }
     }
-    if (cnt > full8) {
+    if (cnt > fullb) {
         float bx[S], by[S], bz[S];
 #pragma unroll
         for (int u = 0; u < S; u++) bx[u] = by[u] = bz[u] = 0.f;
-        for (int q = full8; q < cnt; q++) {
+        for (int q = fullb; q < cnt; q++) {
             const float4 sv = sp[q];
 #pragma unroll
             for (int u = 0; u < S / 2; u++)
@@ -1180,7 +1183,7 @@ __global__ void k_zero_counter(int32_t *c, const int32_t *dnitems, const int32_t
     }
 }
 
-// Pair-math probe: the register-tile loop of k_batched alone (8-source batch
+// Pair-math probe: the register-tile loop of k_batched alone (16-source batch
 // sums, 64-source FP32 tile sums accumulated in FP64), one pair form for
 // every block, on sinks and sources that stay in the caches.  Each wave holds
 // 64 S sinks and makes `passes` passes over `nsrc` sources (a multiple of

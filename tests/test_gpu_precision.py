@@ -2,7 +2,7 @@
 
 Variants: the plain path (FP32 as the reference accumulates it: BLOCK_ACCUM
 tile sums, FP32 offsets, FP32 co-add), the batched path (FP32 sums of
-8-source batches and 64-source tiles, the tile sums accumulated in FP64, FP64
+16-source batches and 64-source tiles, the tile sums accumulated in FP64, FP64
 cell offsets, FP64 co-add; short tails keep TwoSum-compensated FP32 pairs),
 the reference CPU directs (restated, oracle/cpu_directs.c), all against the
 FP64 direct-summation oracle on the same FP32 cell-relative inputs.  Beside
