@@ -237,15 +237,23 @@ __global__ __launch_bounds__(256) void k_gather_sources(DevPlan pl, DevSlabs sl,
 // gained little from sorting them.
 // Cell offsets are added in FP64 (x + dc/cpd, rounded once).
 //
-// One wave per source pencil: the wave loads each of the pencil's binned
-// cells once (two 64-particle chunks each, all cells' loads in flight
-// together), ranks every particle within its cell (ballots per bin), writes
-// it to its slot, and reduces the box of each 64-source block of the pencil
-// over the particles that land in it (the pencil's slots run cell after
-// cell, so its blocks are met in order and each is written once).  The last
-// block's padding slots take kFar.
+// A wave fills a range of a pencil's 64-source blocks: it loads each binned
+// cell that reaches into the range once (two 64-particle chunks each, all
+// cells' loads in flight together), ranks every particle within its cell
+// (ballots per bin), writes the ones whose slots fall in the range, and
+// reduces the box of each block of the range over the particles that land
+// in it (the pencil's slots run cell after cell, so its blocks are met in
+// order).  The last block's padding slots take kFar.
+//
+// One wave per source pencil fills the pencil's first kSegBlocks blocks;
+// one wave per kSegBlocks blocks of the stream fills the blocks that lie
+// beyond the first kSegBlocks of their pencil.  So every block is written
+// by one wave, and a pencil of large cells (a clustered box's halos put
+// thousands of particles in a cell) spreads over many waves instead of
+// holding one for the whole launch.
 constexpr int kXBins = 8;
 constexpr int kBinnedCellMax = 2 * kWave;
+constexpr int kSegBlocks = kGatherSegBlocks;
 
 __device__ __forceinline__ int xbin(float x, float cpdf) {  // x cell-centred; out-of-cell x clamps
     const float t = floorf((x * cpdf + 0.5f) * (float)kXBins);
@@ -289,9 +297,12 @@ struct BlockBox {
     }
 };
 
+// Blocks [blo, bhi) of pencil s (pencil-relative; bhi may pass its end);
+// RANGED false: the whole pencil, without the range tests.
+template <bool RANGED>
 __device__ __forceinline__ void gather_pencil_binned(const DevPlan &pl, const DevSlabs &sl, float4 *__restrict__ out,
                                                      float4 *__restrict__ box, float cpdf, double invcpd, int s,
-                                                     int lane) {
+                                                     int lane, int blo, int bhi) {
     const int W = pl.W;
     const nf_source_plan p = cptr<nf_source_plan>(pl.src_plan)[s];
     int pre[NF_MAX_WIDTH + 1];
@@ -299,19 +310,23 @@ __device__ __forceinline__ void gather_pencil_binned(const DevPlan &pl, const De
 #pragma unroll
     for (int c = 0; c < NF_MAX_WIDTH; c++) pre[c + 1] = pre[c] + (c < W ? p.cell[c].N : 0);
     const int total = pre[NF_MAX_WIDTH];
-    if (total == 0) return;
+    // the slots to fill
+    const int lo = RANGED ? blo * kWave : 0, hi = !RANGED || bhi >= (total + kWave - 1) / kWave ? total : bhi * kWave;
+    if (lo >= hi) return;
     const int start = cptr<int32_t>(pl.src_set_start)[s];  // the pencil's first slot, a multiple of 64
     float4 *dst = out + start;
     const int64_t gb0 = start / kWave;
     const int last = (total - 1) / kWave;
-    if (last * kWave + lane >= total) dst[last * kWave + lane] = make_float4(kFar, 0.f, 0.f, 0.f);
+    if ((!RANGED || (last >= blo && last < bhi)) && last * kWave + lane >= total)
+        dst[last * kWave + lane] = make_float4(kFar, 0.f, 0.f, 0.f);
+    auto overlaps = [&](int c) { return !RANGED || (pre[c] < hi && pre[c + 1] > lo); };
     // the binned cells' particles (two chunks each) all load before any is ranked
     float3 q[NF_MAX_WIDTH][2];
 #pragma unroll
     for (int c = 0; c < NF_MAX_WIDTH; c++) {
         const int n = pre[c + 1] - pre[c];
         q[c][0] = q[c][1] = make_float3(0.f, 0.f, 0.f);
-        if (n > 0 && n <= kBinnedCellMax) {
+        if (n > 0 && n <= kBinnedCellMax && overlaps(c)) {
             const float *P = slab_ptr(sl, c);
             const uint64_t ns = slab_n(sl, c), first = p.cell[c].start;
             if (lane < n) q[c][0] = slab_xyz(P, ns, first + (uint64_t)lane, sl.aos);
@@ -320,11 +335,18 @@ __device__ __forceinline__ void gather_pencil_binned(const DevPlan &pl, const De
     }
     BlockBox bb;
     // The sources at slots s0 (and s1, if in1) of this lane: into the stream
-    // and into the boxes of the blocks from slot lo to slot hi.
-    auto place = [&](int lo, int hi, bool in0, int s0, float3 v0, bool in1, int s1, float3 v1) {
+    // and into the boxes of the blocks from slot a to slot z, as far as they
+    // lie in the range.
+    auto place = [&](int a, int z, bool in0, int s0, float3 v0, bool in1, int s1, float3 v1) {
+        if (RANGED) {
+            in0 = in0 && s0 >= lo && s0 < hi;
+            in1 = in1 && s1 >= lo && s1 < hi;
+            a = max(a, lo);
+            z = min(z, hi - 1);
+        }
         if (in0) dst[s0] = make_float4(v0.x, v0.y, v0.z, 0.f);
         if (in1) dst[s1] = make_float4(v1.x, v1.y, v1.z, 0.f);
-        for (int b = lo / kWave; b <= hi / kWave; b++) {
+        for (int b = a / kWave; b <= z / kWave; b++) {
             bb.to(b, box, gb0, lane);
             bb.add(in0 && s0 / kWave == b, v0.x, v0.y, v0.z);
             bb.add(in1 && s1 / kWave == b, v1.x, v1.y, v1.z);
@@ -336,7 +358,7 @@ __device__ __forceinline__ void gather_pencil_binned(const DevPlan &pl, const De
 #pragma unroll
     for (int c = 0; c < NF_MAX_WIDTH; c++) {
         const int n = pre[c + 1] - pre[c];
-        if (n == 0) continue;
+        if (n == 0 || !overlaps(c)) continue;
         if (n <= kBinnedCellMax) {
             const int bin0 = lane < n ? xbin(q[c][0].x, cpdf) : -1;
             const int bin1 = lane + kWave < n ? xbin(q[c][1].x, cpdf) : -1;
@@ -356,10 +378,12 @@ __device__ __forceinline__ void gather_pencil_binned(const DevPlan &pl, const De
                   pre[c] + rank1, shifted(q[c][1], c));
             continue;
         }
-        // a large cell: the reference's order, 64 at a time
+        // a large cell: the reference's order, 64 at a time (the chunks
+        // that reach into the range)
         const float *P = slab_ptr(sl, c);
         const uint64_t ns = slab_n(sl, c), first = p.cell[c].start;
-        for (int i0 = 0; i0 < n; i0 += kWave) {
+        for (int i0 = RANGED ? max(lo - pre[c], 0) & ~(kWave - 1) : 0; i0 < n && (!RANGED || pre[c] + i0 < hi);
+             i0 += kWave) {
             const bool in = i0 + lane < n;
             const float3 v = in ? slab_xyz(P, ns, first + (uint64_t)(i0 + lane), sl.aos) : make_float3(0.f, 0.f, 0.f);
             place(pre[c] + i0, pre[c] + min(n, i0 + kWave) - 1, in, pre[c] + i0 + lane, shifted(v, c), false, 0,
@@ -369,12 +393,47 @@ __device__ __forceinline__ void gather_pencil_binned(const DevPlan &pl, const De
     bb.flush(box, gb0, lane);
 }
 
+// Wave w of a SIC's source gather: w < nsrc_sets fills pencil w (its first
+// kSegBlocks blocks if the gather is segmented); wave nsrc_sets + j fills,
+// of the stream's blocks [j kSegBlocks, (j + 1) kSegBlocks), those beyond
+// their pencil's first kSegBlocks.  Grid: gather_waves() waves.  SEG false:
+// no SIC of the launch is segmented, and the kernel holds only the
+// whole-pencil code (the segmented code's scalar registers would cost the
+// whole-pencil waves an eighth of their occupancy).
+template <bool SEG>
+__device__ __forceinline__ void gather_wave(const DevPlan &pl, const DevSlabs &sl, float4 *__restrict__ out,
+                                            float4 *__restrict__ box, float cpdf, double invcpd, int w, int lane) {
+    if (!SEG) {
+        gather_pencil_binned<false>(pl, sl, out, box, cpdf, invcpd, w, lane, 0, 0);
+        return;
+    }
+    if (w < pl.nsrc_sets) {
+        if (gather_segmented(pl.max_src_blocks))
+            gather_pencil_binned<true>(pl, sl, out, box, cpdf, invcpd, w, lane, 0, kSegBlocks);
+        else
+            gather_pencil_binned<false>(pl, sl, out, box, cpdf, invcpd, w, lane, 0, 0);
+        return;
+    }
+    // Only the pencil of the segment's first block can reach kSegBlocks past
+    // its own start within the segment: a pencil that starts later in it
+    // ends its first kSegBlocks blocks beyond the segment's end.
+    const int b0 = (w - pl.nsrc_sets) * kSegBlocks;
+    if (b0 >= src_blocks(pl)) return;
+    const int s = cptr<int32_t>(pl.src_parent)[b0];
+    if (s < 0) return;  // a block of no pencil
+    const int f = cptr<int32_t>(pl.src_set_start)[s] / kWave;
+    const int e = f + (cptr<int32_t>(pl.src_set_count)[s] + kWave - 1) / kWave;
+    const int lo = max(b0, f + kSegBlocks), hi = min(b0 + kSegBlocks, e);
+    if (lo < hi) gather_pencil_binned<true>(pl, sl, out, box, cpdf, invcpd, s, lane, lo - f, hi - f);
+}
+
+template <bool SEG>
 __global__ __launch_bounds__(256) void k_gather_sources_binned(DevPlan pl, DevSlabs sl, float4 *__restrict__ out,
                                                                float4 *__restrict__ box, float cpdf,
                                                                double invcpd) {
-    const int s = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
-    if (s >= pl.nsrc_sets || plan_bad(pl)) return;
-    gather_pencil_binned(pl, sl, out, box, cpdf, invcpd, s, threadIdx.x & (kWave - 1));
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x / kWave));
+    if (w >= gather_waves(pl.nsrc_sets, pl.nsrc_blocks, pl.max_src_blocks) || plan_bad(pl)) return;
+    gather_wave<SEG>(pl, sl, out, box, cpdf, invcpd, w, threadIdx.x & (kWave - 1));
 }
 
 // The SIC of a group-wide launch that wave w belongs to: the last i with
@@ -392,6 +451,7 @@ __device__ __forceinline__ int group_sic_of(const int32_t *first, int n, int w) 
 
 // Every SIC's source gather of a launch group in one launch: a group of
 // small SICs (a clustered box's) fills the GPU where one SIC's pencils do not.
+template <bool SEG>
 __global__ __launch_bounds__(256) void k_gather_sources_binned_group(const GroupSic *__restrict__ g,
                                                                      const int32_t *__restrict__ first, int n,
                                                                      float cpdf, double invcpd) {
@@ -402,10 +462,10 @@ __global__ __launch_bounds__(256) void k_gather_sources_binned_group(const Group
     const DevSlabs sl = cptr<GroupSic>(g)[i].sl;
     void *const sources4 = cptr<GroupSic>(g)[i].sources4;
     const int s = w - cptr<int32_t>(first)[i];
-    if (s >= pl.nsrc_sets || plan_bad(pl)) return;
+    if (s >= gather_waves(pl.nsrc_sets, pl.nsrc_blocks, pl.max_src_blocks) || plan_bad(pl)) return;
     float4 *src = static_cast<float4 *>(sources4);
-    gather_pencil_binned(pl, sl, src, src + (size_t)pl.nsrc_blocks * NF_BLOCK_SIZE, cpdf, invcpd, s,
-                         threadIdx.x & (kWave - 1));  // the block boxes follow the stream
+    gather_wave<SEG>(pl, sl, src, src + (size_t)pl.nsrc_blocks * NF_BLOCK_SIZE, cpdf, invcpd, s,
+                     threadIdx.x & (kWave - 1));  // the block boxes follow the stream
 }
 
 // ---------------------------------------------------------------------------
@@ -1302,9 +1362,15 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool batched_stream, vo
     }
     if (pl.nsrc_blocks > 0) {
         const dim3 grid = grid_of(pl.nsrc_blocks);
-        if (batched_stream)
-            hipLaunchKernelGGL(k_gather_sources_binned, dim3((unsigned)((pl.nsrc_sets + 3) / 4)), dim3(256), 0, st,
-                               pl, sl, static_cast<float4 *>(sources4), box, (float)pl.cpd, invcpd);
+        if (batched_stream) {
+            const dim3 bgrid((unsigned)((gather_waves(pl.nsrc_sets, pl.nsrc_blocks, pl.max_src_blocks) + 3) / 4));
+            if (gather_segmented(pl.max_src_blocks))
+                hipLaunchKernelGGL(k_gather_sources_binned<true>, bgrid, dim3(256), 0, st, pl, sl,
+                                   static_cast<float4 *>(sources4), box, (float)pl.cpd, invcpd);
+            else
+                hipLaunchKernelGGL(k_gather_sources_binned<false>, bgrid, dim3(256), 0, st, pl, sl,
+                                   static_cast<float4 *>(sources4), box, (float)pl.cpd, invcpd);
+        }
         else
             hipLaunchKernelGGL(k_gather_sources, grid, dim3(256), 0, st, pl, sl, static_cast<float4 *>(sources4),
                                box, cellsize);
@@ -1312,10 +1378,17 @@ int launch_gather(const DevPlan &pl, const DevSlabs &sl, bool batched_stream, vo
     return (int)hipGetLastError();
 }
 
-int launch_gather_group(const GroupSic *g, const int32_t *first, int n, int total, int cpd, void *stream) {
+int launch_gather_group(const GroupSic *g, const int32_t *first, int n, int total, int cpd, bool segmented,
+                        void *stream) {
     if (total <= 0) return 0;
-    hipLaunchKernelGGL(k_gather_sources_binned_group, dim3((unsigned)((total + 3) / 4)), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), g, first, n, (float)cpd, 1.0 / (double)cpd);
+    const dim3 grid((unsigned)((total + 3) / 4));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (segmented)
+        hipLaunchKernelGGL(k_gather_sources_binned_group<true>, grid, dim3(256), 0, st, g, first, n, (float)cpd,
+                           1.0 / (double)cpd);
+    else
+        hipLaunchKernelGGL(k_gather_sources_binned_group<false>, grid, dim3(256), 0, st, g, first, n, (float)cpd,
+                           1.0 / (double)cpd);
     return (int)hipGetLastError();
 }
 

@@ -15,9 +15,9 @@ namespace nf {
 int fail(int code, const std::string &msg);
 
 // nf_count_interactions' sums (planner.cpp); with strict false a negative
-// count counts as zero.
+// count counts as zero.  max_pencil (or null): the largest source pencil.
 int count_interactions(int32_t cpd, int32_t R, int32_t j_low, int32_t j_high, const nf_cellinfo *const *ci,
-                       bool strict, uint64_t *out);
+                       bool strict, uint64_t *out, int64_t *max_pencil);
 
 // memcpy over nt threads, in 64-byte aligned chunks that cover every byte
 // (hostcopy.cpp); par_memcpy picks nt from the host's cores and the size.
@@ -93,6 +93,7 @@ struct DevPlan {
     int32_t nsink_sets, nsrc_sets, nsink_blocks, nsrc_blocks;
     int32_t W, R, cpd, Nk, j_width, j_low;
     int32_t n_coadd_chunks;
+    int32_t max_src_blocks;  // padded blocks of the largest source pencil (INT32_MAX: not known)
     const GPlanTotals *dtot;
     const int32_t *dbad;
 };
@@ -185,12 +186,25 @@ struct GroupSic {
 // Waves of each SIC in the group-wide launches: the gather takes one per
 // source pencil, the co-add one per 64-particle chunk (its bound, for a
 // device-planned SIC).
-inline int32_t group_gather_waves(const DevPlan &pl) { return pl.nsrc_sets; }
+// Waves of a SIC's binned source gather: one per source pencil and, when a
+// pencil may have more than kGatherSegBlocks blocks, one per
+// kGatherSegBlocks blocks of the stream (the pencil waves then fill their
+// first kGatherSegBlocks blocks, these the blocks past them).
+constexpr int32_t kGatherSegBlocks = 8;
+constexpr bool gather_segmented(int32_t max_src_blocks) { return max_src_blocks > kGatherSegBlocks; }
+constexpr int32_t gather_waves(int32_t nsrc_sets, int32_t nsrc_blocks, int32_t max_src_blocks) {
+    return nsrc_sets + (gather_segmented(max_src_blocks) ? (nsrc_blocks + kGatherSegBlocks - 1) / kGatherSegBlocks : 0);
+}
+inline int32_t group_gather_waves(const DevPlan &pl) {
+    return gather_waves(pl.nsrc_sets, pl.nsrc_blocks, pl.max_src_blocks);
+}
 inline int32_t group_coadd_waves(const DevPlan &pl) { return pl.n_coadd_chunks; }
 // The batched-path source gathers of n SICs as one launch, then (a second
 // call) their FP64 co-adds as one launch.  first[i] is the first wave of SIC
 // i and first[n] the total (device memory, like g).
-int launch_gather_group(const GroupSic *g, const int32_t *first, int n, int total, int cpd, void *stream);
+// segmented: some SIC of the group has gather_segmented(max_src_blocks).
+int launch_gather_group(const GroupSic *g, const int32_t *first, int n, int total, int cpd, bool segmented,
+                        void *stream);
 int launch_coadd_group(const GroupSic *g, const int32_t *first, int n, int total, void *stream);
 // The single-spline pair-math probe (the batched tile loop alone, one pair
 // form: 0 unsoftened without density, 1 with density, 2 the full spline) at

@@ -323,9 +323,11 @@ extern "C" int nf_split_slab(int32_t cpd, int32_t R, const nf_cellinfo *const *c
 namespace nf {
 
 // The separable cube sums behind nf_count_interactions; negative counts are
-// an error (strict) or count as zero.
+// an error (strict) or count as zero.  max_pencil (or null) receives the
+// particles of the SIC's largest source pencil: the first sums, one per
+// (y, z) over the W slabs, are exactly the source pencils' counts.
 int count_interactions(int32_t cpd, int32_t R, int32_t j_low, int32_t j_high, const nf_cellinfo *const *ci,
-                       bool strict, uint64_t *out) {
+                       bool strict, uint64_t *out, int64_t *max_pencil) {
     const int W = 2 * R + 1, rows = (j_high - j_low) + 2 * R;
     auto wrap = [&](int i) { return ((i % cpd) + cpd) % cpd; };
     // m: particles of the W slabs in each cell of the source rows j_low-R ..
@@ -353,6 +355,7 @@ int count_interactions(int32_t cpd, int32_t R, int32_t j_low, int32_t j_high, co
             run += mr[wrap(z + R + 1)] - mr[wrap(z - R)];
         }
     }
+    if (max_pencil) *max_pencil = m.empty() ? 0 : *std::max_element(m.begin(), m.end());
     uint64_t total = 0;
     for (int j = j_low; j < j_high; j++) {
         const size_t row = (size_t)j * cpd;
@@ -381,5 +384,5 @@ extern "C" int nf_count_interactions(int32_t cpd, int32_t nfradius, int32_t j_lo
         return nf::fail(NF_ERR_INVALID, "nf_count_interactions: bad sink row range");
     for (int c = 0; c < 2 * nfradius + 1; c++)
         if (!cellinfo[c]) return nf::fail(NF_ERR_INVALID, "nf_count_interactions: missing cellinfo slab");
-    return nf::count_interactions(cpd, nfradius, j_low, j_high, cellinfo, true, out);
+    return nf::count_interactions(cpd, nfradius, j_low, j_high, cellinfo, true, out, nullptr);
 }

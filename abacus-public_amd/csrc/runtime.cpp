@@ -95,6 +95,7 @@ struct Task {
         uint64_t chunks_bound = 0, items_bound = 0;  // bounds the plan is laid out for
         uint64_t occ[4] = {};     // sinks in pencils of <64, 64-255, 256-1023, >=1024 sinks
         uint64_t interactions = 0;  // DirectTotal from the cellinfo counts (nf_count_interactions)
+        int32_t max_src_blocks = INT32_MAX;  // padded blocks of the largest source pencil, from the same sums
     } dp;
     std::atomic<int> state{0};  // 0 queued/running, 1 done, <0 failed
     std::string error;
@@ -381,6 +382,14 @@ DevPlan dev_plan(const nf_sic &s, const PlanLayout &L, char *pd) {
     pl.nsrc_sets = s.NSourceSets;
     pl.nsink_blocks = s.NSinkBlocks;
     pl.nsrc_blocks = s.NSourceBlocks;
+    // (a SIC planned on the device has no host counts: its kernels read the
+    // largest pencil from the device totals)
+    pl.max_src_blocks = INT32_MAX;
+    if (s.SourceSetCount) {
+        pl.max_src_blocks = 0;
+        for (int k = 0; k < s.NSourceSets; k++)
+            pl.max_src_blocks = std::max(pl.max_src_blocks, (s.SourceSetCount[k] + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE);
+    }
     pl.W = s.nfwidth;
     pl.R = s.nfradius;
     pl.cpd = s.cpd;
@@ -1180,6 +1189,7 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
         DevPlan &pl = plans[i];
         pl = dev_plan(s, L, pd);
         if (dplan) {  // the actual counts, in device memory; the group's bad cells
+            pl.max_src_blocks = g[i]->dp.max_src_blocks;
             char *sc = static_cast<char *>(w.gscratch.p);
             pl.dtot = reinterpret_cast<const GPlanTotals *>(sc + 256 * (size_t)i);
             pl.dbad = reinterpret_cast<const int32_t *>(sc + gs.o_shared) + kPlanGroupBad;
@@ -1355,8 +1365,11 @@ int run_group(Runtime &rt, Worker &w, std::vector<Task *> &g) {
     int e = 0;
     const GroupSic *gsic_d = reinterpret_cast<const GroupSic *>(pd + o_gsic);
     const int32_t *gfirst_d = reinterpret_cast<const int32_t *>(pd + o_gfirst), *cfirst_d = gfirst_d + (n + 1);
-    if (path == NF_PATH_BATCHED)
-        e = launch_gather_group(gsic_d, gfirst_d, n, gfirst[n], s0.cpd, stream);
+    if (path == NF_PATH_BATCHED) {
+        bool seg = false;
+        for (int i = 0; i < n; i++) seg = seg || gather_segmented(plans[i].max_src_blocks);
+        e = launch_gather_group(gsic_d, gfirst_d, n, gfirst[n], s0.cpd, seg, stream);
+    }
     else
         for (int i = 0; i < n && !e; i++)
             e = launch_gather(plans[i], slabs[i], false, wb + WL[i].o_sinks, wb + WL[i].o_src, stream);
@@ -1900,8 +1913,12 @@ extern "C" int nf_submit_slab(const nf_slab_task *p, int32_t blocking, nf_task *
     for (int c = 0; c < W; c++) t->dp.cellinfo[c] = p->cellinfo[c];
     // The SIC's exact pair count sizes its launch group and the heavy-tile
     // split before the device plans it (a bad count fails the task in phase A).
-    if (count_interactions(p->cpd, R, p->j_low, p->j_high, p->cellinfo, false, &t->dp.interactions) != NF_OK)
+    int64_t max_pencil = 0;
+    if (count_interactions(p->cpd, R, p->j_low, p->j_high, p->cellinfo, false, &t->dp.interactions, &max_pencil) !=
+        NF_OK)
         t->dp.interactions = 0;
+    else
+        t->dp.max_src_blocks = (int32_t)std::min<int64_t>(INT32_MAX, (max_pencil + NF_BLOCK_SIZE - 1) / NF_BLOCK_SIZE);
     // The sink rows run from cell (j_low, 0) to the end of cell (j_high-1,
     // cpd-1); phase A checks both cells against the slab.
     const nf_cellinfo &first = p->cellinfo[R][(size_t)p->j_low * p->cpd];
