@@ -184,6 +184,56 @@ def test_empty_and_heavy_cells():
             assert st["rms"] < 3e-6, (path, x, st)
 
 
+def test_heavy_pencils_split_gather():
+    """Source pencils of more than 8 blocks (512 particles) have their
+    batched-path gather split over several waves: the pencil's own wave fills
+    its first 8 blocks, one wave per 8 blocks of the stream the rest.  Cells
+    of 511, 512, 513, 1,100 and 3,000 particles (uniform within their cells)
+    make pencils on both sides of that edge.  The group gather (nf_submit,
+    host plans), the device-planned path (nf_submit_slab) and the per-SIC
+    gather of nf_execute_sic_noalloc must give bitwise the same
+    accelerations, and those must meet the FP64 oracle: a block written
+    twice or not at all would miss or double 64 sources of a sink's sum."""
+    cpd = 7
+    rng = np.random.default_rng(29)
+    w = 1.0 / cpd
+    centre = lambda i: (np.asarray(i) + 0.5) * w - 0.5
+    heavy = {(3, 1, 1): 511, (3, 1, 4): 512, (3, 4, 2): 513, (2, 5, 5): 1100, (3, 3, 6): 3000}
+    bg = rng.random((cpd ** 3 * 20, 3)) - 0.5
+    bg_cell = np.floor((bg + 0.5) * cpd).astype(int)
+    pts = [bg[~np.isin(bg_cell @ [cpd * cpd, cpd, 1], [x * cpd * cpd + y * cpd + z for x, y, z in heavy])]]
+    for cell, n in heavy.items():
+        pts.append(centre(cell) + (rng.random((n, 3)) - 0.5) * w * 0.999)
+    box = synth.box_from_global(np.concatenate(pts), cpd)
+    for (x, y, z), n in heavy.items():
+        assert box.slab(x).count[y * cpd + z] == n, (x, y, z)
+    eps, b2, _ = params(box)
+    slabs = [1, 3, 5]
+    ell = box.mean_spacing()
+    with nfm.NearFieldDriver(cpd, ell / 40.0, b2, nfradius=R, path="batched", buffers_per_device=2,
+                             max_buffer_bytes=1 << 30) as drv:
+        host = run_sics(drv, box, slabs, nfm.NF_MEM_DEVICE, planner="host")
+        dev = run_sics(drv, box, slabs, nfm.NF_MEM_DEVICE, planner="device")
+        # the per-SIC gather: slab 3 through nf_execute_sic_noalloc
+        pos = [nfm.DeviceArray(host=box.slab(x).pos) for x in range(cpd)]
+        acc = nfm.DeviceArray(host=np.full((box.slab(3).n, 4), np.nan, np.float32))
+        try:
+            sic = drv.plan(box, 3, pos_ptrs=[pos[(3 - R + c) % cpd].ptr for c in range(2 * R + 1)],
+                           acc_ptr=acc.ptr, mem=nfm.NF_MEM_DEVICE)
+            assert sic.arrays()["SourceSetCount"].max() > 8 * 64  # the gather is split
+            nfm.execute_sic_noalloc(sic)
+            noalloc = acc.to_host(np.float32, (box.slab(3).n, 4))
+        finally:
+            for d in pos + [acc]:
+                d.free()
+    assert np.array_equal(noalloc.view(np.uint32), host[3].view(np.uint32))
+    for x in slabs:
+        assert np.array_equal(host[x].view(np.uint32), dev[x].view(np.uint32)), x
+        ref, soft = oracle_slab(box, x, "single_spline", eps, b2, with_soft=True)
+        st = summarize(rel_err(host[x], ref), not_soft_dominated(ref, soft))
+        assert st["rms"] < 1e-6 and st["max"] < 1e-4, (x, st)
+
+
 def test_batched_source_order_regimes():
     """The batched path orders each source cell's particles by x-bin: cells of
     up to 64 particles (one chunk), 65-128 (two chunks), and larger cells kept
