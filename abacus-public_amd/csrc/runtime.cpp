@@ -1581,7 +1581,11 @@ bool compatible(const nf_sic &a, const nf_sic &b) {
     return a.softening == b.softening && a.eps == b.eps && a.b2 == b.b2 && a.mem == b.mem && a.cpd == b.cpd;
 }
 
-constexpr int kGroupFillWaitUs = 1000;
+// How long a worker waits for a full group while earlier work is in flight
+// (so the wait never leaves the GPU idle).  On a clustered 256^3 box 2 ms
+// gave fuller groups than 1 ms (11.2 launches per step against 12, +0.6%)
+// and 0.5 ms fewer; a uniform box's groups fill at once either way.
+constexpr int kGroupFillWaitUs = 2000;
 
 void worker_main(Runtime *rt, Worker *w) {
     for (;;) {
